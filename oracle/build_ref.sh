@@ -1,0 +1,31 @@
+#!/usr/bin/env bash
+# Build the reference checkers from the reference's own sources, where they lie.
+#   oracle/_ref/librefpoint.so : the reference's inline EOS / flux / eigen / WENO kernels
+#                                (include/Equations.hpp, include/Utility.hpp) behind C entry
+#                                points (oracle/ref_point.cpp)
+#   oracle/_ref/refsolver      : the reference's unmodified 4-rank solver (src/WENOFV.cpp)
+#                                driven by oracle/ref_solver.cpp; needs the MPI found in
+#                                this development container (/opt/conda)
+# Flags follow the reference makefile (-O3, no -march: no FMA).  Outputs go only to
+# oracle/_ref/ (git-ignored).  Development container only: the reference is not present
+# on the GPU machines, where only the committed fixtures are used.
+set -euo pipefail
+REF=${WENOFV_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+[ -d "$REF/include" ] || { echo "reference not present at $REF; skipping"; exit 0; }
+mkdir -p "$OUT"
+CXXFLAGS="-std=c++17 -O3 -w -I$REF/include -I$REF/third_party/scTools"
+g++ $CXXFLAGS -fPIC -shared "$HERE/ref_point.cpp" -o "$OUT/librefpoint.so"
+echo "built $OUT/librefpoint.so"
+MPI_INC=${MPI_INC:-/opt/conda/include}
+MPI_LIB=${MPI_LIB:-/opt/conda/lib}
+if [ -f "$MPI_INC/mpi.h" ] && [ -f "$MPI_LIB/libmpi.so" ]; then
+  g++ $CXXFLAGS -I"$MPI_INC" -c "$REF/src/WENOFV.cpp" -o "$OUT/WENOFV.o"
+  g++ $CXXFLAGS -I"$MPI_INC" -c "$HERE/ref_solver.cpp" -o "$OUT/ref_solver.o"
+  g++ -o "$OUT/refsolver" "$OUT/ref_solver.o" "$OUT/WENOFV.o" "$MPI_LIB/libmpi.so" \
+      -Wl,-rpath,/usr/lib/x86_64-linux-gnu:"$MPI_LIB"
+  echo "built $OUT/refsolver"
+else
+  echo "MPI not found; refsolver not built"
+fi

@@ -1,0 +1,317 @@
+"""Write the golden fixtures of the WENOFV path from the compiled reference.
+
+Development container only: it needs the reference checkers built from the reference's
+own sources by oracle/build_ref.sh (oracle/_ref/librefpoint.so, oracle/_ref/refsolver,
+the latter run under MPICH's mpirun with the reference's hard-coded 4 ranks).  The
+fixtures it writes (tests/golden/*.npz) are data: inputs and the reference's outputs.
+
+    python tests/golden/make_golden.py [--only NAME ...]
+"""
+import argparse
+import ctypes
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+MPIRUN = os.environ.get("WENOFV_MPIRUN", "/opt/conda/bin/mpirun")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# reference test cases (include/Equations.hpp:486-977) used by the fixtures
+CASES = {"SMOOTH": 0, "VORTEX": 1, "SHARP": 2, "DAMBREAK": 3, "DAMBREAK2": 4, "DAMBREAK3": 5,
+         "RTI": 6, "STANDING_WAVE": 7, "ZALESAK": 8}
+# additional parameter sets of the liquid-column / tanh workloads (rho10, rho20, c1, c2, preb)
+EXTRA_PARAMS = [(1.0, 0.1, 10.0, 10.0, 0.0), (1.0, 1e-3, 10.0, 10.0, 0.0), (1.0, 1e-2, 10.0, 10.0, 0.0)]
+
+
+def ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def refpoint():
+    lib = ctypes.CDLL(os.path.join(REF_DIR, "librefpoint.so"))
+    lib.ref_set_params.argtypes = [ctypes.c_double] * 5
+    lib.ref_set_case.argtypes = [ctypes.c_int, _dp]
+    for n in ("ref_weno_js", "ref_weno_z"):
+        getattr(lib, n).restype = ctypes.c_double
+        getattr(lib, n).argtypes = [_dp, ctypes.c_int]
+    for n in ("ref_volume_frac", "ref_pressure", "ref_sound_speed"):
+        getattr(lib, n).restype = ctypes.c_double
+        getattr(lib, n).argtypes = [_dp]
+    lib.ref_max_eig.restype = ctypes.c_double
+    lib.ref_max_eig.argtypes = [_dp, ctypes.c_double, ctypes.c_double]
+    lib.ref_phy_flux.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double]
+    lib.ref_riemann.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    lib.ref_eigen.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    lib.ref_u0.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
+    return lib
+
+
+def weno_stencils(rng):
+    """Stencil classes: random, polynomial cell averages, steps, near-constant, scales."""
+    out = [rng.standard_normal((200, 5))]
+    # cell averages of random quartics over cells [k-1/2, k+1/2], k=-2..2
+    x = np.arange(-2, 3, dtype=np.float64)
+    for _ in range(60):
+        c = rng.standard_normal(5)
+        avg = [sum(c[m] * (((xk + 0.5) ** (m + 1) - (xk - 0.5) ** (m + 1)) / (m + 1)) for m in range(5)) for xk in x]
+        out.append(np.array(avg)[None, :])
+    steps = []
+    for k in range(1, 5):
+        for lo, hi in ((0.0, 1.0), (1.0, 0.0), (1e-8, 1 - 1e-8), (1000.0, 1.0)):
+            s = np.full(5, lo)
+            s[k:] = hi
+            steps.append(s)
+    out.append(np.array(steps))
+    out.append(1.0 + 1e-9 * rng.standard_normal((40, 5)))
+    out.append(1e-8 * rng.random((40, 5)))
+    out.append(1e3 * rng.standard_normal((40, 5)))
+    out.append(np.zeros((1, 5)))
+    out.append(np.full((1, 5), 3.25))
+    return np.ascontiguousarray(np.vstack(out))
+
+
+def states_for(params, rng, n):
+    """Valid conservative states: z1 spread over (eps, 1-eps), pressure around the base."""
+    rho10, rho20, c1, c2, preb = params
+    z1 = np.concatenate([10.0 ** rng.uniform(-8, -0.3, n // 2), 1 - 10.0 ** rng.uniform(-8, -0.3, n - n // 2)])
+    prs = preb + rng.uniform(-0.3, 2.0, n) * max(rho20 * c2 * c2, 1e-3) * 0.05 + rng.uniform(0, 1, n)
+    r1 = rho10 + (prs - preb) / (c1 * c1)
+    r2 = rho20 + (prs - preb) / (c2 * c2)
+    u = rng.uniform(-2, 2, n)
+    v = rng.uniform(-2, 2, n)
+    z1r1, z2r2 = z1 * r1, (1 - z1) * r2
+    rho = z1r1 + z2r2
+    return np.ascontiguousarray(np.stack([z1r1, z2r2, rho * u, rho * v], axis=1))
+
+
+def make_point(rng):
+    lib = refpoint()
+    S = weno_stencils(rng)
+    wjs = np.array([[lib.ref_weno_js(ptr(np.ascontiguousarray(s)), g) for g in range(4)] for s in S])
+    wz = np.array([[lib.ref_weno_z(ptr(np.ascontiguousarray(s)), g) for g in range(4)] for s in S])
+    np.savez_compressed(os.path.join(HERE, "point_weno.npz"), stencils=S, js=wjs, z=wz)
+
+    psets = []
+    out5 = np.zeros(5)
+    for name, tc in CASES.items():
+        lib.ref_set_case(tc, ptr(out5))
+        psets.append(out5.copy())
+    psets += [np.array(p) for p in EXTRA_PARAMS]
+    psets = np.array(psets)
+    n = 40
+    allU, eos, eig, flx, LR = [], [], [], [], []
+    for p in psets:
+        lib.ref_set_params(*p)
+        U = states_for(p, rng, n)
+        for u in U:
+            u = np.ascontiguousarray(u)
+            fx, fy = np.zeros(4), np.zeros(4)
+            lib.ref_phy_flux(ptr(u), ptr(fx), 1.0, 0.0)
+            lib.ref_phy_flux(ptr(u), ptr(fy), 0.0, 1.0)
+            Lx, Rx, Ly, Ry = (np.zeros(16) for _ in range(4))
+            lib.ref_eigen(ptr(u), 1.0, 0.0, ptr(Lx), ptr(Rx))
+            lib.ref_eigen(ptr(u), 0.0, 1.0, ptr(Ly), ptr(Ry))
+            eos.append([lib.ref_volume_frac(ptr(u)), lib.ref_pressure(ptr(u)), lib.ref_sound_speed(ptr(u))])
+            eig.append([lib.ref_max_eig(ptr(u), 1.0, 0.0), lib.ref_max_eig(ptr(u), 0.0, 1.0)])
+            flx.append(np.concatenate([fx, fy]))
+            LR.append(np.stack([Lx, Rx, Ly, Ry]))
+        allU.append(U)
+    np.savez_compressed(os.path.join(HERE, "point_eos.npz"), params=psets, U=np.array(allU),
+                        eos=np.array(eos).reshape(len(psets), n, 3), eig=np.array(eig).reshape(len(psets), n, 2),
+                        flux=np.array(flx).reshape(len(psets), n, 8), LR=np.array(LR).reshape(len(psets), n, 4, 16))
+
+    # Riemann pairs: random neighbours, contacts (equal u and p), strong jumps, both normals
+    UL, UR, F = [], [], []
+    for p in psets:
+        lib.ref_set_params(*p)
+        A = states_for(p, rng, 24)
+        B = states_for(p, rng, 24)
+        pairs = [(A[k], B[k]) for k in range(16)]
+        for k in range(4):  # near neighbours
+            pairs.append((A[16 + k], A[16 + k] * (1 + 1e-3 * rng.standard_normal(4))))
+        for k in range(4):  # contact: same velocity and pressure, different z1
+            rho10, rho20, c1, c2, preb = p
+            pr = preb + 0.5
+            r1, r2 = rho10 + (pr - preb) / c1 ** 2, rho20 + (pr - preb) / c2 ** 2
+            zs = (1e-6, 1 - 1e-6) if k % 2 == 0 else (0.3, 0.7)
+            st = []
+            for z in zs:
+                rho = z * r1 + (1 - z) * r2
+                st.append(np.array([z * r1, (1 - z) * r2, rho * 0.7, rho * -0.2]))
+            pairs.append(tuple(st))
+        for ul, ur in pairs:
+            ul, ur = np.ascontiguousarray(ul), np.ascontiguousarray(ur)
+            row = []
+            for t in (0, 1):
+                for (nx, ny) in ((1.0, 0.0), (0.0, 1.0)):
+                    f = np.zeros(4)
+                    lib.ref_riemann(ptr(ul), ptr(ur), ptr(f), nx, ny, t)
+                    row.append(f)
+            UL.append(ul)
+            UR.append(ur)
+            F.append(np.array(row))
+    npair = len(UL) // len(psets)
+    np.savez_compressed(os.path.join(HERE, "point_riemann.npz"), params=psets,
+                        UL=np.array(UL).reshape(len(psets), npair, 4), UR=np.array(UR).reshape(len(psets), npair, 4),
+                        F=np.array(F).reshape(len(psets), npair, 4, 4))
+    print("point fixtures written")
+
+
+def run_ref(args, tmp, state=None, nx=None, ny=None):
+    """Run refsolver under 4 MPI ranks in a scratch directory; returns the output prefix."""
+    state_path = "-"
+    if state is not None:
+        state_path = os.path.join(tmp, "state_in.bin")
+        np.ascontiguousarray(state, dtype=np.float64).tofile(state_path)
+    prefix = os.path.join(tmp, "out")
+    cmd = [MPIRUN, "-np", "4", os.path.join(REF_DIR, "refsolver")] + [str(a) for a in args[:10]] + \
+          [str(args[10]), state_path, prefix] + [repr(float(a)) for a in args[11:]]
+    # a reference run that meets a NaN pauses on std::cin and its peers spin in MPI:
+    # bound every run and treat a timeout as a failed combination
+    res = subprocess.run(cmd, cwd=tmp, stdin=subprocess.DEVNULL, capture_output=True, text=True,
+                         timeout=120)
+    if res.returncode != 0:
+        raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
+    return prefix, res.stdout
+
+
+def load(prefix, suffix, shape=None):
+    a = np.fromfile(prefix + suffix, dtype=np.float64)
+    return a.reshape(shape) if shape else a
+
+
+def random_state(rng, nx, ny, params, kind):
+    rho10, rho20, c1, c2, preb = params
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    if kind == "smooth":
+        z1 = 0.5 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.05 * rng.standard_normal((ny, nx))
+        pr = preb + 0.5 + 0.1 * rng.standard_normal((ny, nx))
+        u = 0.5 + 0.2 * rng.standard_normal((ny, nx))
+        v = -0.3 + 0.2 * rng.standard_normal((ny, nx))
+    else:  # sharp: near-pure phases with jumps (activates the bound-preserving limiter)
+        eps = 1e-8
+        z1 = np.where((np.abs(X - 0.5) < 0.2) & (np.abs(Y - 0.45) < 0.25), 1 - eps, eps)
+        z1 = np.where(rng.random((ny, nx)) < 0.1, 1 - z1, z1)
+        pr = preb + 1.0 + 0.2 * rng.random((ny, nx))
+        u = 1.0 + 0.5 * rng.standard_normal((ny, nx))
+        v = 1.0 + 0.5 * rng.standard_normal((ny, nx))
+    r1 = rho10 + (pr - preb) / c1 ** 2
+    r2 = rho20 + (pr - preb) / c2 ** 2
+    z1r1, z2r2 = z1 * r1, (1 - z1) * r2
+    rho = z1r1 + z2r2
+    return np.stack([z1r1, z2r2, rho * u, rho * v])
+
+
+def make_rhs(rng):
+    nx = ny = 12
+    out = {}
+    for kind, params in (("smooth", (1.2, 1.0, 1.0, 1.0, 0.0)), ("sharp", (1000.0, 1.0, 70.0, 70.0, 0.0))):
+        U0 = random_state(rng, nx, ny, params, kind)
+        out[f"{kind}_U"] = U0
+        out[f"{kind}_params"] = np.array(params)
+        for scheme in (0, 1):
+            for recon in (0, 1, 2, 3):
+                for flux in (0, 1):
+                    for bp in (0, 1):
+                        if kind == "sharp" and (bp == 0 or recon >= 2):
+                            continue  # negative partial densities without the limiter: NaN in the reference
+                        tag = f"{kind}_s{scheme}_r{recon}_f{flux}_b{bp}"
+                        with tempfile.TemporaryDirectory() as tmp:
+                            prefix, _ = run_ref(["rhs", 0, nx, ny, 2, 0.4, scheme, bp, flux, recon, 0.0, *params],
+                                                tmp, state=U0)
+                            init = load(prefix, "_init.bin", (4, ny, nx))
+                            assert np.array_equal(init, U0)
+                            out[tag + "_rhs"] = load(prefix, "_rhs.bin", (4, ny, nx))
+                            if recon in (0, 1) and (scheme == bp or kind == "sharp"):
+                                out[tag + "_faces"] = load(prefix, "_faces.bin", (ny, nx, 64))
+                            out[tag + "_thflag"] = load(prefix, "_thetaflag.bin", (ny, nx))
+    np.savez_compressed(os.path.join(HERE, "solver_rhs.npz"), **out)
+    print("rhs fixtures written")
+
+
+def make_theta(rng):
+    """Cells built around partial-density violations; reference getPositivityPreservingTheta."""
+    recs = []
+    for k in range(64):
+        params = (1000.0, 1.0, 70.0, 70.0, 0.0) if k % 2 else (1.2, 1.0, 1.0, 1.0, 0.0)
+        U = states_for(params, rng, 1)[0]
+        faces = np.tile(U[None, :, None], (4, 1, 4)) * (1 + 0.3 * rng.standard_normal((4, 4, 4)))
+        if k % 4 < 2:  # push some partial-density face values below zero
+            faces[rng.integers(4), rng.integers(2), rng.integers(4)] = -abs(U[0]) * rng.random()
+        recs.append((params, np.concatenate([U, faces.reshape(-1)])))
+    out = {}
+    for params in ((1.2, 1.0, 1.0, 1.0, 0.0), (1000.0, 1.0, 70.0, 70.0, 0.0)):
+        R = np.array([r for p, r in recs if p == params])
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "cells.bin")
+            R.tofile(path)
+            prefix = os.path.join(tmp, "out")
+            cmd = [MPIRUN, "-np", "4", os.path.join(REF_DIR, "refsolver"), "theta", "0", "16", "16", "2", "0.4",
+                   "0", "1", "0", "0", str(len(R)), path, prefix] + [repr(float(a)) for a in params]
+            subprocess.run(cmd, cwd=tmp, stdin=subprocess.DEVNULL, capture_output=True, check=True)
+            th = load(prefix, "_theta.bin")
+        key = "c1" if params[2] == 1.0 else "c70"
+        out[key + "_cells"] = R
+        out[key + "_theta"] = th
+        out[key + "_params"] = np.array(params)
+    # grid used: TESTCASE 0 domain [0,1]^2 at 16x16
+    out["dX"] = np.array(1.0 / 16)
+    out["dY"] = np.array(1.0 / 16)
+    np.savez_compressed(os.path.join(HERE, "point_theta.npz"), **out)
+    print("theta fixtures written")
+
+
+RUNS = [
+    # name, testcase, nx, ny, rk, cfl, scheme, bp, flux, recon, mode, t_end|steps
+    ("smooth32_js_llf_cons_rk3", 0, 32, 32, 2, 0.4, 0, 0, 0, 0, "run", 0.05),
+    ("smooth32_z_hllc_char_bp_rk3", 0, 32, 32, 2, 0.4, 1, 1, 1, 1, "run", 0.05),
+    ("smooth20_js_hllc_prim_rk2", 0, 20, 20, 1, 0.5, 0, 0, 1, 2, "run", 0.02),
+    ("smooth20_z_llf_prim2_rk1", 0, 20, 20, 0, 0.3, 1, 1, 0, 3, "run", 0.01),
+    ("vortex32_z_hllc_char_bp_rk3", 1, 32, 32, 2, 0.4, 1, 1, 1, 1, "run", 0.005),
+    ("sharp32_z_hllc_cons_bp_rk3", 2, 32, 32, 2, 0.4, 1, 1, 1, 0, "steps", 30),
+    ("sharp32_js_llf_char_bp_rk3", 2, 32, 32, 2, 0.4, 0, 1, 0, 1, "steps", 30),
+    ("smooth100_z_llf_char_rk3_cfg", 0, 100, 100, 2, 0.5, 1, 0, 0, 1, "steps", 5),
+]
+
+
+def make_runs():
+    out = {}
+    for name, tc, nx, ny, rk, cfl, scheme, bp, flux, recon, mode, tend in RUNS:
+        with tempfile.TemporaryDirectory() as tmp:
+            prefix, so = run_ref([mode, tc, nx, ny, rk, cfl, scheme, bp, flux, recon, tend], tmp)
+            out[name + "_init"] = load(prefix, "_init.bin", (4, ny, nx))
+            out[name + "_final"] = load(prefix, "_final.bin", (4, ny, nx))
+            out[name + "_dts"] = load(prefix, "_dts.bin")
+            out[name + "_opts"] = np.array([tc, nx, ny, rk, cfl, scheme, bp, flux, recon, 1 if mode == "steps" else 0,
+                                            tend], dtype=np.float64)
+            print(name, so.strip().splitlines()[-1])
+    np.savez_compressed(os.path.join(HERE, "solver_runs.npz"), **out)
+    print("run fixtures written")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", nargs="*", default=None)
+    a = ap.parse_args()
+    if not os.path.exists(os.path.join(REF_DIR, "refsolver")):
+        sys.exit("oracle/_ref not built: run oracle/build_ref.sh first")
+    rng = np.random.default_rng(20261203)
+    jobs = {"point": lambda: make_point(rng), "rhs": lambda: make_rhs(rng), "theta": lambda: make_theta(rng),
+            "runs": make_runs}
+    for k, f in jobs.items():
+        if a.only is None or k in a.only:
+            f()
+
+
+if __name__ == "__main__":
+    main()
