@@ -1,0 +1,116 @@
+/*
+ * wenofv.h -- C ABI of the MI355X (gfx950) WENOFV time-stepping library (libwenofv.so).
+ *
+ * The reference has no plugin or FFI layer: its boundary is the C++ class CWENOFV
+ * (include/WENOFV.hpp:108-257 of the reference) configured by the key = value options of
+ * config/settings.cfg.  Each entry point below replaces one CWENOFV method (cited per
+ * function) for a single domain on one GPU; the C++ facade (csrc/WENOFV.hpp) restores the
+ * class interface on top of it.
+ *
+ * Conventions.  Host arrays are caller-owned; the library copies them.  A state array is
+ * double[4][ny][nx] (SoA, x fastest) holding (z1*rho1, z2*rho2, rho*u, rho*v).  Every
+ * function returns a WFV_* status; wfv_last_error() describes the last failure.  A context
+ * is not thread-safe; distinct contexts are independent.  Nothing here exits or reads stdin.
+ */
+#ifndef WENOFV_H
+#define WENOFV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define WFV_OK 0
+#define WFV_ERR_CONFIG 1        /* invalid option (reference: std::stoi throw, MPI_Abort) */
+#define WFV_ERR_HIP 2           /* HIP runtime failure                                     */
+#define WFV_ERR_NONFINITE_DT 3  /* NaN CFL time step (reference: exit(1), WENOFV.cpp:502-531) */
+#define WFV_ERR_ARG 4           /* null pointer / bad size                                 */
+
+/* error bits accumulated on the device (wfv_get_flags) */
+#define WFV_FLAG_NAN_FLUX 1u   /* a Riemann flux was NaN (reference prints: Equations.hpp:344-359) */
+#define WFV_FLAG_BAD_THETA 2u  /* limiter theta outside [0,1] (reference prints: WENOFV.cpp:1927)  */
+#define WFV_FLAG_NAN_DT 4u     /* a NaN CFL time step                                              */
+
+/* Solver options: the config/settings.cfg keys (reference config/settings.cfg:11-24, read in
+   src/WENOFV.cpp:43-63) plus the options of this implementation (defaults reproduce the
+   reference). */
+typedef struct wfv_options {
+    int elem_num_x;      /* ELEMNUMX */
+    int elem_num_y;      /* ELEMNUMY */
+    int rk_method;       /* RKMETH: 0 SSP-RK1, 1 SSP-RK2, 2 SSP-RK3 */
+    double cfl;          /* CFL */
+    int scheme;          /* SCHEME: 0 WENO-JS, 1 WENO-Z */
+    int pp_limiter;      /* PPlimiter: bound-preserving limiter on the partial densities */
+    int riemann_flux;    /* RIEMANNFLUX: 0 LLF, 1 HLLC */
+    int reconstruction;  /* RECONSTRUCTIONTYPE: 0 conservative, 1 characteristic,
+                            2 primitive (z1, rho1, u, v), 3 primitive2 (z1, p, u, v) */
+    int precision;       /* 0: all-FP64, bit-faithful to the reference; 1: mixed precision */
+    int boundary;        /* 0 periodic (the reference's only boundary), 1 reflective walls */
+    int geometry_ranks;  /* cell widths as the reference's N-rank slab grid builds them (4),
+                            or 1 for a single-domain grid */
+    int rows_per_strip;  /* rows one wavefront marches (0: tuned default) */
+} wfv_options;
+
+/* Physical case: domain and TwoPhaseEquation parameters (reference include/Equations.hpp:61-70). */
+typedef struct wfv_case {
+    double xL, xR, yL, yR;
+    double rho10, rho20;   /* g_rho10, g_rho20 */
+    double sound1, sound2; /* g_sound1, g_sound2 */
+    double preb;           /* g_preb */
+    double gravity;        /* source term rhs[3] -= rho*g (assembleSourceTerm); 0: none */
+} wfv_case;
+
+typedef struct wfv_ctx wfv_ctx;
+
+/* CWENOFV::CWENOFV + initializeSolver (include/WENOFV.hpp:114-118, src/WENOFV.cpp:16-230):
+   validates the options, builds the grid and allocates the three RK registers on the
+   current HIP device. */
+int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out);
+void wfv_destroy(wfv_ctx *ctx);
+const char *wfv_last_error(const wfv_ctx *ctx);
+
+/* Cell averages in / out (the reference's m_Uh over interior cells, getWorldUh:
+   src/WENOFV.cpp:2059-2117).  The _device forms take device pointers. */
+int wfv_set_state(wfv_ctx *ctx, const double *U);
+int wfv_get_state(wfv_ctx *ctx, double *U);
+int wfv_set_state_device(wfv_ctx *ctx, const double *dU);
+int wfv_get_state_device(wfv_ctx *ctx, double *dU);
+
+/* CWENOFV::calculateDeltaT (src/WENOFV.cpp:474-540): min(1, min_cells CFL/(ax/dx+ay/dy)). */
+int wfv_compute_dt(wfv_ctx *ctx, double *dt);
+
+/* CWENOFV::performTimeIntegration (src/WENOFV.cpp:541-566): one SSP-RK step of size dt. */
+int wfv_step(wfv_ctx *ctx, double dt);
+
+/* CWENOFV::assembleRHS (src/WENOFV.cpp:568-615) of the current state.  rhs: [4][ny][nx];
+   faces (optional): limited face values per cell [ny][nx][side][var][gp] with side order
+   left, right, bottom, top (the reference's m_gridFlux); theta (optional): [ny][nx]. */
+int wfv_assemble_rhs(wfv_ctx *ctx, double *rhs, double *faces, double *theta);
+
+/* CWENOFV::run's time loop (src/WENOFV.cpp:318-332) on the device: from t_start, steps of
+   dt = min(CFL dt, t_end - now) until |now - t_end| <= 1e-10, or exactly fixed_steps
+   unclipped CFL steps when fixed_steps > 0.  Steps are replayed from a HIP graph without
+   host round trips.  dts (optional, host, max_dts entries) receives each dt. */
+int wfv_advance(wfv_ctx *ctx, double t_start, double t_end, int fixed_steps, int *nsteps,
+                double *t_out, double *dts, int max_dts);
+
+/* Error bits raised by the kernels since the last call (WFV_FLAG_*); clears them. */
+int wfv_get_flags(wfv_ctx *ctx, unsigned *flags);
+int wfv_synchronize(wfv_ctx *ctx);
+
+/* Run the context's work on a caller-owned HIP stream (hipStream_t passed as void*; NULL
+   restores the context's own stream).  Lets a caller order the solver with its own work
+   and time it with events on that stream.  No reference counterpart (the reference has no
+   device). */
+int wfv_set_stream(wfv_ctx *ctx, void *hip_stream);
+
+/* Grid of the context: interior cells and the per-column / per-row widths. */
+int wfv_grid(const wfv_ctx *ctx, int *nx, int *ny, double *dxC, double *dxW, double *dyC);
+
+/* Library build identification (kernel variants compiled for gfx950). */
+const char *wfv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

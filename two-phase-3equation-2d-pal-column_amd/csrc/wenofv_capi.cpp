@@ -1,0 +1,399 @@
+// wenofv_capi.cpp -- C ABI of the single-domain WENOFV solver (include/wenofv.h).
+//
+// Host side of the time-stepping path: option validation, grid geometry, the three RK
+// registers in HBM, stage scheduling and the device-resident time loop replayed from a
+// HIP graph.  All arithmetic runs in the kernels of wenofv_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wenofv.h"
+#include "wenofv_geometry.h"
+#include "wenofv_kernels.h"
+
+using namespace wfv;
+
+struct wfv_ctx {
+    wfv_options opt{};
+    wfv_case cs{};
+    int nx = 0, ny = 0;
+    size_t nc = 0;
+    double *reg[3] = {nullptr, nullptr, nullptr};  // RK registers; reg[0] is the state
+    double *d_dxC = nullptr, *d_dxW = nullptr, *d_dyC = nullptr;
+    double *d_faces = nullptr, *d_theta = nullptr;
+    StepCtl *d_ctl = nullptr;
+    double *d_dts = nullptr;
+    int dts_cap = 0;
+    hipStream_t stream = nullptr;     // stream in use
+    hipStream_t own_stream = nullptr; // stream created with the context
+    Phys P{};
+    double dX = 0, dY = 0;
+    int ty = 16;
+    std::vector<double> dxC, dxW, dyC;
+    std::string err;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+};
+
+namespace {
+
+thread_local std::string g_create_error;  // reported by wfv_last_error(NULL)
+
+int fail(wfv_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, WFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+StageArgs base_args(wfv_ctx *c) {
+    StageArgs A{};
+    A.dxC = c->d_dxC;
+    A.dxW = c->d_dxW;
+    A.dyC = c->d_dyC;
+    A.P = c->P;
+    A.dX = c->dX;
+    A.dY = c->dY;
+    A.cfl = c->opt.cfl;
+    A.gravity = c->cs.gravity;
+    A.nx = c->nx;
+    A.ny = c->ny;
+    A.bc = c->opt.boundary;
+    A.flux = c->opt.riemann_flux;
+    A.bp = c->opt.pp_limiter;
+    A.ty = c->ty;
+    A.strips_x = (c->nx + strip_width() - 1) / strip_width();
+    A.strips_y = (c->ny + c->ty - 1) / c->ty;
+    A.flags = &c->d_ctl->flags;
+    return A;
+}
+
+// RK schedule of SSP-RK1/2/3 (src/WENOFV.cpp:376-473): per stage (a, b, c) and registers
+struct StagePlan {
+    double a, b, c;
+    int in, base, out;  // register indices; base < 0: Un = U
+};
+int rk_plan(int rk, StagePlan *s) {
+    if (rk == 0) {
+        // U += dt*rhs, written as (0*U + 1*U) + (1*dt)*rhs, which rounds identically
+        s[0] = {0.0, 1.0, 1.0, 0, -1, 1};
+        return 1;
+    }
+    if (rk == 1) {
+        s[0] = {1.0, 0.0, 1.0, 0, -1, 1};
+        s[1] = {0.5, 0.5, 0.5, 1, 0, 0};
+        return 2;
+    }
+    s[0] = {1.0, 0.0, 1.0, 0, -1, 1};
+    s[1] = {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, 1, 0, 2};
+    s[2] = {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 2, 0, 0};
+    return 3;
+}
+
+// enqueue one RK step; dt from the host (dt_dev == nullptr) or from the device loop
+int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active, bool want_dtmin) {
+    StagePlan pl[3];
+    const int ns = rk_plan(c->opt.rk_method, pl);
+    for (int s = 0; s < ns; ++s) {
+        StageArgs A = base_args(c);
+        A.U = c->reg[pl[s].in];
+        A.Un = pl[s].base < 0 ? nullptr : c->reg[pl[s].base];
+        A.Uo = c->reg[pl[s].out];
+        A.a = pl[s].a;
+        A.b = pl[s].b;
+        A.c = pl[s].c;
+        A.dt = dt;
+        A.dt_ptr = dt_dev;
+        A.active = active;
+        const bool last = s == ns - 1;
+        A.dtmin = (last && want_dtmin && c->opt.rk_method != 0) ? &c->d_ctl->dtmin_bits : nullptr;
+        HIPCHK(c, launch_stage(A, c->opt.precision, c->opt.reconstruction, c->opt.scheme, c->stream));
+    }
+    if (c->opt.rk_method == 0) {
+        // RK1 writes register 1; copy back so that register 0 stays the state
+        HIPCHK(c, hipMemcpyAsync(c->reg[0], c->reg[1], 4 * c->nc * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream));
+        if (want_dtmin) {
+            DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags,
+                     &c->d_ctl->dtmin_bits};
+            HIPCHK(c, launch_dt(D, c->stream));
+        }
+    }
+    return WFV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *wfv_version(void) { return "wenofv-gfx950 1.0 (fp64, mixed; js, z; cons, char, prim, prim2)"; }
+
+int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
+    if (!opt || !cs || !out) return WFV_ERR_ARG;
+    *out = nullptr;
+    wfv_ctx *c = new wfv_ctx();
+    c->opt = *opt;
+    c->cs = *cs;
+    std::string why;
+    if (opt->elem_num_x < 1 || opt->elem_num_y < 1) why = "ELEMNUMX/ELEMNUMY must be positive";
+    else if (opt->elem_num_x < 3 || opt->elem_num_y < 3) why = "grids need at least 3 cells per direction";
+    else if (opt->rk_method < 0 || opt->rk_method > 2) why = "RKMETH must be 0, 1 or 2";
+    else if (opt->scheme < 0 || opt->scheme > 1) why = "SCHEME must be 0 (WENO-JS) or 1 (WENO-Z)";
+    else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
+    else if (opt->reconstruction < 0 || opt->reconstruction > 3) why = "RECONSTRUCTIONTYPE must be 0..3";
+    else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 (fp64) or 1 (mixed)";
+    else if (opt->boundary < 0 || opt->boundary > 1) why = "boundary must be 0 (periodic) or 1 (walls)";
+    else if (!(opt->cfl > 0)) why = "CFL must be positive";
+    else if (opt->geometry_ranks < 0 || (opt->geometry_ranks > 1 && opt->elem_num_x % opt->geometry_ranks))
+        why = "ELEMNUMX must be divisible by the number of ranks of the reference grid";
+    else if (!(cs->xR > cs->xL) || !(cs->yR > cs->yL)) why = "empty domain";
+    if (!why.empty()) {
+        delete c;
+        g_create_error = why;
+        return WFV_ERR_CONFIG;
+    }
+    c->nx = opt->elem_num_x;
+    c->ny = opt->elem_num_y;
+    c->nc = (size_t)c->nx * c->ny;
+    c->ty = opt->rows_per_strip > 0 ? opt->rows_per_strip : default_rows_per_strip(c->nx, c->ny);
+    c->P.rho10 = cs->rho10;
+    c->P.rho20 = cs->rho20;
+    c->P.c1_2 = cs->sound1 * cs->sound1;
+    c->P.c2_2 = cs->sound2 * cs->sound2;
+    c->P.preb = cs->preb;
+    c->P.q = cs->rho20 * c->P.c2_2 - cs->rho10 * c->P.c1_2;
+    c->dX = (cs->xR - cs->xL) / c->nx;
+    c->dY = (cs->yR - cs->yL) / c->ny;
+    c->dxC.resize(c->nx);
+    c->dxW.resize(c->nx);
+    c->dyC.resize(c->ny);
+    grid_widths(c->nx, c->ny, cs->xL, cs->xR, cs->yL, cs->yR, opt->geometry_ranks < 1 ? 1 : opt->geometry_ranks,
+                c->dxC.data(), c->dxW.data(), c->dyC.data(), nullptr, nullptr);
+    auto bad = [&](hipError_t e) {
+        if (e == hipSuccess) return false;
+        g_create_error = hipGetErrorString(e);
+        wfv_destroy(c);
+        return true;
+    };
+    if (bad(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking))) return WFV_ERR_HIP;
+    c->stream = c->own_stream;
+    for (int k = 0; k < 3; ++k)
+        if (bad(hipMalloc(&c->reg[k], 4 * c->nc * sizeof(double)))) return WFV_ERR_HIP;
+    if (bad(hipMalloc(&c->d_dxC, c->nx * sizeof(double))) || bad(hipMalloc(&c->d_dxW, c->nx * sizeof(double))) ||
+        bad(hipMalloc(&c->d_dyC, c->ny * sizeof(double))) || bad(hipMalloc(&c->d_ctl, sizeof(StepCtl))))
+        return WFV_ERR_HIP;
+    if (bad(hipMemcpy(c->d_dxC, c->dxC.data(), c->nx * sizeof(double), hipMemcpyHostToDevice)) ||
+        bad(hipMemcpy(c->d_dxW, c->dxW.data(), c->nx * sizeof(double), hipMemcpyHostToDevice)) ||
+        bad(hipMemcpy(c->d_dyC, c->dyC.data(), c->ny * sizeof(double), hipMemcpyHostToDevice)))
+        return WFV_ERR_HIP;
+    StepCtl init{};
+    init.dtmin_bits = ~0ull;
+    init.active = 1;
+    if (bad(hipMemcpy(c->d_ctl, &init, sizeof init, hipMemcpyHostToDevice))) return WFV_ERR_HIP;
+    if (bad(hipMemset(c->reg[0], 0, 4 * c->nc * sizeof(double)))) return WFV_ERR_HIP;
+    *out = c;
+    return WFV_OK;
+}
+
+void wfv_destroy(wfv_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    for (auto &p : c->reg) hipFree(p);
+    hipFree(c->d_dxC);
+    hipFree(c->d_dxW);
+    hipFree(c->d_dyC);
+    hipFree(c->d_faces);
+    hipFree(c->d_theta);
+    hipFree(c->d_ctl);
+    hipFree(c->d_dts);
+    if (c->own_stream) {
+        (void)hipStreamSynchronize(c->own_stream);
+        (void)hipStreamDestroy(c->own_stream);
+    }
+    delete c;
+}
+
+const char *wfv_last_error(const wfv_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int wfv_set_state(wfv_ctx *c, const double *U) {
+    if (!c || !U) return WFV_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(c->reg[0], U, 4 * c->nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WFV_OK;
+}
+int wfv_get_state(wfv_ctx *c, double *U) {
+    if (!c || !U) return WFV_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(U, c->reg[0], 4 * c->nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WFV_OK;
+}
+int wfv_set_state_device(wfv_ctx *c, const double *dU) {
+    if (!c || !dU) return WFV_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(c->reg[0], dU, 4 * c->nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WFV_OK;
+}
+int wfv_get_state_device(wfv_ctx *c, double *dU) {
+    if (!c || !dU) return WFV_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(dU, c->reg[0], 4 * c->nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WFV_OK;
+}
+
+int wfv_compute_dt(wfv_ctx *c, double *dt) {
+    if (!c || !dt) return WFV_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    StepCtl h{};
+    HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+    h.dtmin_bits = ~0ull;
+    HIPCHK(c, hipMemcpy(c->d_ctl, &h, sizeof h, hipMemcpyHostToDevice));
+    DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags, &c->d_ctl->dtmin_bits};
+    HIPCHK(c, launch_dt(D, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+    if (h.flags & WFV_FLAG_NAN_DT) return fail(c, WFV_ERR_NONFINITE_DT, "Error: exist nan DeltaT");
+    double m;
+    std::memcpy(&m, &h.dtmin_bits, sizeof m);
+    double timestep = 1.0;  // src/WENOFV.cpp:477, 510-513
+    if (h.dtmin_bits != ~0ull && !(m > timestep)) timestep = m;
+    *dt = timestep;
+    return WFV_OK;
+}
+
+int wfv_step(wfv_ctx *c, double dt) {
+    if (!c) return WFV_ERR_ARG;
+    int rc = enqueue_step(c, dt, nullptr, nullptr, false);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WFV_OK;
+}
+
+int wfv_assemble_rhs(wfv_ctx *c, double *rhs, double *faces, double *theta) {
+    if (!c || !rhs) return WFV_ERR_ARG;
+    if ((faces || theta) && !c->d_faces) {
+        HIPCHK(c, hipMalloc(&c->d_faces, c->nc * 64 * sizeof(double)));
+        HIPCHK(c, hipMalloc(&c->d_theta, c->nc * sizeof(double)));
+    }
+    StageArgs A = base_args(c);
+    A.U = c->reg[0];
+    A.Un = nullptr;
+    A.Uo = c->reg[2];
+    A.a = 0.0;  // Uo = (0*U + 0*U) + (1*1)*rhs = rhs
+    A.b = 0.0;
+    A.c = 1.0;
+    A.dt = 1.0;
+    A.faces_out = (faces || theta) ? c->d_faces : nullptr;
+    A.theta_out = (faces || theta) ? c->d_theta : nullptr;
+    HIPCHK(c, launch_stage(A, c->opt.precision, c->opt.reconstruction, c->opt.scheme, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(rhs, c->reg[2], 4 * c->nc * sizeof(double), hipMemcpyDeviceToHost));
+    if (faces) HIPCHK(c, hipMemcpy(faces, c->d_faces, c->nc * 64 * sizeof(double), hipMemcpyDeviceToHost));
+    if (theta) HIPCHK(c, hipMemcpy(theta, c->d_theta, c->nc * sizeof(double), hipMemcpyDeviceToHost));
+    return WFV_OK;
+}
+
+int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *nsteps, double *t_out,
+                double *dts, int max_dts) {
+    if (!c) return WFV_ERR_ARG;
+    if (dts && max_dts > c->dts_cap) {
+        hipFree(c->d_dts);
+        c->d_dts = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_dts, (size_t)max_dts * sizeof(double)));
+        c->dts_cap = max_dts;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    StepCtl h{};
+    HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+    h.dtmin_bits = ~0ull;
+    h.now = t_start;
+    h.t_end = t_end;
+    h.count = 0;
+    h.fixed_steps = fixed_steps;
+    h.active = 1;
+    h.dts = dts ? c->d_dts : nullptr;
+    h.max_record = dts ? max_dts : 0;
+    HIPCHK(c, hipMemcpy(c->d_ctl, &h, sizeof h, hipMemcpyHostToDevice));
+    // CFL minimum of the initial state
+    DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags, &c->d_ctl->dtmin_bits};
+    HIPCHK(c, launch_dt(D, c->stream));
+    // a batch of steps captured once and replayed; steps past the end of the loop exit at once
+    const int batch = fixed_steps > 0 ? (fixed_steps < 32 ? fixed_steps : 32) : 32;
+    if (!c->graph || c->graph_steps != batch) {
+        if (c->graph) hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = WFV_OK;
+        for (int k = 0; k < batch && rc == WFV_OK; ++k) {
+            if (launch_step_control(c->d_ctl, c->stream) != hipSuccess) rc = WFV_ERR_HIP;
+            if (rc == WFV_OK) rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true);
+        }
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != WFV_OK) return rc;
+        if (e != hipSuccess) return fail(c, WFV_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(c, WFV_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+        c->graph_steps = batch;
+    }
+    for (;;) {
+        HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+        if (!h.active) break;
+    }
+    if (nsteps) *nsteps = h.count;
+    if (t_out) *t_out = h.now;
+    if (dts && h.count > 0) {
+        const int n = h.count < max_dts ? h.count : max_dts;
+        HIPCHK(c, hipMemcpy(dts, c->d_dts, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (h.flags & WFV_FLAG_NAN_DT) return fail(c, WFV_ERR_NONFINITE_DT, "Error: exist nan DeltaT");
+    return WFV_OK;
+}
+
+int wfv_get_flags(wfv_ctx *c, unsigned *flags) {
+    if (!c || !flags) return WFV_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    StepCtl h{};
+    HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+    *flags = h.flags;
+    h.flags = 0;
+    HIPCHK(c, hipMemcpy(c->d_ctl, &h, sizeof h, hipMemcpyHostToDevice));
+    return WFV_OK;
+}
+
+int wfv_set_stream(wfv_ctx *c, void *hip_stream) {
+    if (!c) return WFV_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stream = hip_stream ? (hipStream_t)hip_stream : c->own_stream;
+    return WFV_OK;
+}
+
+int wfv_synchronize(wfv_ctx *c) {
+    if (!c) return WFV_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return WFV_OK;
+}
+
+int wfv_grid(const wfv_ctx *c, int *nx, int *ny, double *dxC, double *dxW, double *dyC) {
+    if (!c) return WFV_ERR_ARG;
+    if (nx) *nx = c->nx;
+    if (ny) *ny = c->ny;
+    if (dxC) std::memcpy(dxC, c->dxC.data(), c->nx * sizeof(double));
+    if (dxW) std::memcpy(dxW, c->dxW.data(), c->nx * sizeof(double));
+    if (dyC) std::memcpy(dyC, c->dyC.data(), c->ny * sizeof(double));
+    return WFV_OK;
+}
+
+}  // extern "C"

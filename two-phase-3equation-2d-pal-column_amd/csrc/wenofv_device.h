@@ -1,0 +1,323 @@
+// wenofv_device.h -- device physics and reconstruction kernels of the WENOFV stage.
+//
+// Two precision policies share this code:
+//   PrecFP64  : every operation in double, in the reference's association order, with
+//               IEEE division and square root; compiled with -ffp-contract=off this
+//               reproduces the reference CPU solver bit for bit.
+//   PrecMixed : WENO smoothness indicators, nonlinear weights and flux arithmetic in
+//               float with hardware reciprocals; the equation of state (volume
+//               fraction, pressure, sound speed), the limiter's theta and the RK
+//               accumulation stay in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "weno_constants.h"
+
+#define WFV_DEV __device__ __forceinline__
+
+namespace wfv {
+
+struct PrecFP64 {
+    using W = double;  // WENO arithmetic and the y-sweep values kept in LDS
+    using F = double;  // Riemann-flux arithmetic and face values
+    static constexpr bool exact = true;
+};
+struct PrecMixed {
+    using W = float;
+    using F = float;
+    static constexpr bool exact = false;
+};
+
+// per-case constants; the host fills them with the reference's own expressions
+// (c_k^2 = g_sound_k * g_sound_k, q = rho20*c2^2 - rho10*c1^2, include/Equations.hpp:86-92)
+struct Phys {
+    double rho10, rho20, c1_2, c2_2, preb, q;
+};
+
+// Gauss-Lobatto-4 weights (third_party/scTools/GaussQuadratureInRef.hpp:252-262)
+constexpr double GLW[4] = {0.166666666666666666666666666667, 0.833333333333333333333333333333,
+                                               0.833333333333333333333333333333, 0.166666666666666666666666666667};
+
+template <class T>
+WFV_DEV T dv(T a, T b) {  // division: IEEE in double; hardware reciprocal in float
+    if constexpr (sizeof(T) == 8) {
+        return a / b;
+    } else {
+        return a * __builtin_amdgcn_rcpf(b);
+    }
+}
+WFV_DEV double stdMin(double a, double b) { return (b < a) ? b : a; }  // std::min
+WFV_DEV double stdMax(double a, double b) { return (a < b) ? b : a; }  // std::max
+WFV_DEV float stdMin(float a, float b) { return (b < a) ? b : a; }
+WFV_DEV float stdMax(float a, float b) { return (a < b) ? b : a; }
+WFV_DEV double scMin(double a, double b) { return a < b ? a : b; }      // sc_math::Min
+
+// ------------------------------------------------------------------ equation of state
+// TwoPhaseEquation::getVolumeFrac / getPressure / getSoundSpeed (include/Equations.hpp:84-146)
+// evaluated once per state and shared by every use, in double in both policies.
+struct Eos {
+    double rho, z1, rho1, rho2, p, c;
+};
+WFV_DEV void eos(const Phys &P, const double U[4], Eos &e) {
+    const double z1rho1 = U[0], z2rho2 = U[1];
+    const double qs = z2rho2 * P.c2_2 - z1rho1 * P.c1_2;
+    const double dq = P.q - qs;
+    const double r = (dq + sqrt(dq * dq + 4.0 * z1rho1 * P.c1_2 * z2rho2 * P.c2_2)) / (2.0 * z2rho2 * P.c2_2);
+    const double z1 = r / (1.0 + r);
+    const double z2 = 1.0 - z1;
+    e.rho = z1rho1 + z2rho2;
+    e.z1 = z1;
+    e.rho1 = z1rho1 / z1;
+    e.rho2 = z2rho2 / z2;
+    const double pre1 = P.c1_2 * (e.rho1 - P.rho10);
+    const double pre2 = P.c2_2 * (e.rho2 - P.rho20);
+    e.p = (z1 < z2) ? pre2 : pre1;
+    const double sc = sqrt(e.rho * (z1 / (e.rho1 * P.c1_2) + z2 / (e.rho2 * P.c2_2)));
+    e.c = 1.0 / sc;
+}
+
+// ------------------------------------------------------------------ Riemann fluxes
+// F = face-normal flux through (NX, NY) in {(1,0), (0,1)}.  The literal products with
+// the normal's components are kept so that the double policy matches the reference's
+// rounding and signed zeros.
+template <class T, int NX, int NY>
+WFV_DEV void phy_flux(const T U[4], T rho, T u, T v, T p, T F[4]) {
+    // TwoPhaseEquation::getPhyFlux (include/Equations.hpp:158-186)
+    if constexpr (NX > NY) {
+        F[0] = U[0] * u; F[1] = U[1] * u; F[2] = rho * (u * u) + p; F[3] = rho * u * v;
+    } else {
+        F[0] = U[0] * v; F[1] = U[1] * v; F[2] = rho * v * u; F[3] = rho * (v * v) + p;
+    }
+}
+
+template <class T, int NX, int NY>
+WFV_DEV void riemann_llf(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
+    // getLLFRiemannFlux (include/Equations.hpp:363-378)
+    double dl[4] = {(double)UL[0], (double)UL[1], (double)UL[2], (double)UL[3]};
+    double dr[4] = {(double)UR[0], (double)UR[1], (double)UR[2], (double)UR[3]};
+    Eos el, er;
+    eos(P, dl, el);
+    eos(P, dr, er);
+    const T nx = T(NX), ny = T(NY);
+    const T rl = T(el.rho), rr = T(er.rho);
+    const T ul = dv(UL[2], rl), vl = dv(UL[3], rl);
+    const T ur = dv(UR[2], rr), vr = dv(UR[3], rr);
+    const T wsL = fabs(ul * nx + vl * ny) + T(el.c);
+    const T wsR = fabs(ur * nx + vr * ny) + T(er.c);
+    const T ws = stdMax(wsL, wsR);
+    T FL[4], FR[4];
+    phy_flux<T, NX, NY>(UL, rl, ul, vl, T(el.p), FL);
+    phy_flux<T, NX, NY>(UR, rr, ur, vr, T(er.p), FR);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) F[r] = T(0.5) * (FL[r] + FR[r] - ws * (UR[r] - UL[r]));
+}
+
+template <class T, int NX, int NY>
+WFV_DEV void riemann_hllc(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
+    // getHLLCRiemannFlux (include/Equations.hpp:381-483)
+    double dl[4] = {(double)UL[0], (double)UL[1], (double)UL[2], (double)UL[3]};
+    double dr[4] = {(double)UR[0], (double)UR[1], (double)UR[2], (double)UR[3]};
+    Eos el, er;
+    eos(P, dl, el);
+    eos(P, dr, er);
+    const T nx = T(NX), ny = T(NY);
+    const T rho_L = T(el.rho), rho_R = T(er.rho);
+    const T u_L = dv(UL[2], rho_L), v_L = dv(UL[3], rho_L);
+    const T u_R = dv(UR[2], rho_R), v_R = dv(UR[3], rho_R);
+    const T pre_L = T(el.p), pre_R = T(er.p), c_L = T(el.c), c_R = T(er.c);
+    const T vn_L = u_L * nx + v_L * ny;
+    const T vn_R = u_R * nx + v_R * ny;
+    T FL[4], FR[4];
+    FL[0] = UL[0] * vn_L; FL[1] = UL[1] * vn_L;
+    FL[2] = UL[2] * vn_L + pre_L * nx; FL[3] = UL[3] * vn_L + pre_L * ny;
+    FR[0] = UR[0] * vn_R; FR[1] = UR[1] * vn_R;
+    FR[2] = UR[2] * vn_R + pre_R * nx; FR[3] = UR[3] * vn_R + pre_R * ny;
+    const T sL = sqrt(rho_L), sR = sqrt(rho_R);
+    const T vn_roe = dv(vn_L * sL + vn_R * sR, sL + sR);
+    T c2_roe = dv(c_L * c_L * sL + c_R * c_R * sR, sL + sR);
+    c2_roe += dv(dv(T(0.5) * sL * sR, sL + sR), sL + sR) * (vn_R - vn_L) * (vn_R - vn_L);
+    const T c_roe = sqrt(c2_roe);
+    const T S_L = stdMin(vn_L - c_L, vn_roe - c_roe);
+    const T S_R = stdMax(vn_R + c_R, vn_roe + c_roe);
+    const T S_star = dv((S_L - vn_L) * rho_L * vn_L - (S_R - vn_R) * rho_R * vn_R + pre_R - pre_L,
+                        (S_L - vn_L) * rho_L - (S_R - vn_R) * rho_R);
+    if (S_star >= T(0)) {
+        if (S_L >= T(0)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) F[r] = FL[r];
+        } else {
+            const T k = dv(S_L - vn_L, S_L - S_star);
+            T Us[4] = {UL[0], UL[1], rho_L * (u_L + (S_star - vn_L) * nx), rho_L * (v_L + (S_star - vn_L) * ny)};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) F[r] = S_L * (k * Us[r] - UL[r]) + FL[r];
+        }
+    } else {
+        if (S_R <= T(0)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) F[r] = FR[r];
+        } else {
+            const T k = dv(S_R - vn_R, S_R - S_star);
+            T Us[4] = {UR[0], UR[1], rho_R * (u_R + (S_star - vn_R) * nx), rho_R * (v_R + (S_star - vn_R) * ny)};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) F[r] = S_R * (k * Us[r] - UR[r]) + FR[r];
+        }
+    }
+}
+
+template <class T, int NX, int NY>
+WFV_DEV void riemann(const Phys &P, int type, const T UL[4], const T UR[4], T F[4]) {
+    // getRiemannFlux dispatch (include/Equations.hpp:329-360)
+    if (type == 1) riemann_hllc<T, NX, NY>(P, UL, UR, F);
+    else riemann_llf<T, NX, NY>(P, UL, UR, F);
+}
+
+// ------------------------------------------------------------------ eigenvectors
+// getLEigenMatrix / getREigenMatrix (include/Equations.hpp:189-327), row-major 4x4,
+// built in double (they involve the equation of state) and used in W.
+template <class T, bool XDIR>
+WFV_DEV void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
+    Eos e;
+    eos(P, U, e);
+    const double rho = e.rho, u = U[2] / rho, v = U[3] / rho, c = e.c;
+    const double z1 = e.z1, z2 = 1.0 - z1;
+    const double zrho1 = U[0] * P.c1_2 / (z1 * z1);
+    const double zrho2 = U[1] * P.c2_2 / (z2 * z2);
+    const double sc1_2 = P.c1_2 / z1 - P.c1_2 / z1 * zrho1 / (zrho1 + zrho2);
+    const double sc2_2 = P.c2_2 / z2 - P.c2_2 / z2 * zrho2 / (zrho1 + zrho2);
+    const double c2 = c * c;
+    const double d12 = sc1_2 - sc2_2;
+    const double un = XDIR ? u : v;
+    L[0] = T((sc1_2 + c * un) / (2.0 * c2));
+    L[1] = T((sc2_2 + c * un) / (2.0 * c2));
+    L[4] = T(-U[1] * d12 / (rho * c2));
+    L[5] = T(U[0] * d12 / (rho * c2));
+    L[6] = T(0); L[7] = T(0);
+    L[12] = T((sc1_2 - c * un) / (2.0 * c2));
+    L[13] = T((sc2_2 - c * un) / (2.0 * c2));
+    if (XDIR) {
+        L[2] = T(-1.0 / (2.0 * c)); L[3] = T(0);
+        L[8] = T(-v); L[9] = T(-v); L[10] = T(0); L[11] = T(1);
+        L[14] = T(1.0 / (2.0 * c)); L[15] = T(0);
+    } else {
+        L[2] = T(0); L[3] = T(-1.0 / (2.0 * c));
+        L[8] = T(-u); L[9] = T(-u); L[10] = T(1); L[11] = T(0);
+        L[14] = T(0); L[15] = T(1.0 / (2.0 * c));
+    }
+    R[0] = T(U[0] / rho); R[4] = T(U[1] / rho);
+    R[1] = T(-sc2_2 / d12); R[5] = T(sc1_2 / d12); R[9] = T(u); R[13] = T(v);
+    R[2] = T(0); R[6] = T(0);
+    R[3] = T(U[0] / rho); R[7] = T(U[1] / rho);
+    if (XDIR) {
+        R[8] = T(u - c); R[12] = T(v);
+        R[10] = T(0); R[14] = T(1);
+        R[11] = T(u + c); R[15] = T(v);
+    } else {
+        R[8] = T(u); R[12] = T(v - c);
+        R[10] = T(1); R[14] = T(0);
+        R[11] = T(u); R[15] = T(v + c);
+    }
+}
+
+// Array2D * Array1D (third_party/scTools/Array.h:604-626): C[i] = ((0 + A_i0 B0) + ...) + A_i3 B3
+template <class T>
+WFV_DEV void matvec(const T A[16], const T B[4], T C[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        T acc = T(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc += A[i * 4 + j] * B[j];
+        C[i] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ WENO
+// One 5-point stencil {a,b,c,d,e} = {u_mm, u_m, u, u_p, u_pp}; the smoothness indicators
+// are evaluated once and shared by every Gauss-Lobatto node reconstructed from the
+// stencil (the reference recomputes them per node with identical operands).
+template <class T>
+WFV_DEV void beta3(T a, T b, T c, T d, T e, T &b0, T &b1, T &b2) {
+    // include/Utility.hpp:48-50 (and 270-272 for WENO-Z on the normalised stencil)
+    T t;
+    t = a - T(4) * b + T(3) * c; b0 = T(0.25) * (t * t);
+    t = a - T(2) * b + c;        b0 = b0 + T(13.0 / 12.0) * (t * t);
+    t = b - d;                   b1 = T(0.25) * (t * t);
+    t = b - T(2) * c + d;        b1 = b1 + T(13.0 / 12.0) * (t * t);
+    t = T(3) * c - T(4) * d + e; b2 = T(0.25) * (t * t);
+    t = c - T(2) * d + e;        b2 = b2 + T(13.0 / 12.0) * (t * t);
+}
+
+template <class T, int SCHEME>
+struct Weno5 {
+    T s0, s1, s2, s3, s4;
+    T k0, k1, k2;  // JS: (eps+beta_k)^2, or its reciprocal in float; Z: tau/(beta_k+eps)
+
+    WFV_DEV Weno5(T a, T b, T c, T d, T e) : s0(a), s1(b), s2(c), s3(d), s4(e) {
+        T b0, b1, b2;
+        if constexpr (SCHEME == 0) {
+            // WENO5threconstruction (include/Utility.hpp:48-61), eps = 1e-6
+            beta3(a, b, c, d, e, b0, b1, b2);
+            const T eps = T(1.0e-6);
+            k0 = (eps + b0) * (eps + b0);
+            k1 = (eps + b1) * (eps + b1);
+            k2 = (eps + b2) * (eps + b2);
+            if constexpr (sizeof(T) == 4) {
+                k0 = __builtin_amdgcn_rcpf(k0);
+                k1 = __builtin_amdgcn_rcpf(k1);
+                k2 = __builtin_amdgcn_rcpf(k2);
+            }
+        } else {
+            // WENO5Zthreconstruction (include/Utility.hpp:261-280): beta of the min-max
+            // normalised stencil, tau = |beta0 - beta2|, eps = 1e-10
+            const T mx = stdMax(stdMax(stdMax(a, b), stdMax(e, d)), c);
+            const T mn = stdMin(stdMin(stdMin(a, b), stdMin(e, d)), c);
+            const T den = mx - mn + T(1e-20);
+            T a_, b_, c_, d_, e_;
+            if constexpr (sizeof(T) == 8) {
+                a_ = (a - mn) / den; b_ = (b - mn) / den; c_ = (c - mn) / den;
+                d_ = (d - mn) / den; e_ = (e - mn) / den;
+            } else {
+                const T id = __builtin_amdgcn_rcpf(den);
+                a_ = (a - mn) * id; b_ = (b - mn) * id; c_ = (c - mn) * id;
+                d_ = (d - mn) * id; e_ = (e - mn) * id;
+            }
+            beta3(a_, b_, c_, d_, e_, b0, b1, b2);
+            const T eps = T(1.0e-10);
+            const T tau = fabs(b0 - b2);
+            k0 = dv(tau, b0 + eps);
+            k1 = dv(tau, b1 + eps);
+            k2 = dv(tau, b2 + eps);
+        }
+    }
+
+    template <int GP>
+    WFV_DEV T node() const {
+        const T h0 = T(WENO_H[GP][0][0]) * s0 + T(WENO_H[GP][0][1]) * s1 + T(WENO_H[GP][0][2]) * s2;
+        const T h1 = T(WENO_H[GP][1][0]) * s1 + T(WENO_H[GP][1][1]) * s2 + T(WENO_H[GP][1][2]) * s3;
+        const T h2 = T(WENO_H[GP][2][0]) * s2 + T(WENO_H[GP][2][1]) * s3 + T(WENO_H[GP][2][2]) * s4;
+        T w0, w1, w2;
+        if constexpr (SCHEME == 0) {
+            if constexpr (sizeof(T) == 8) {
+                w0 = T(WENO_D[GP][0]) / k0;
+                w1 = T(WENO_D[GP][1]) / k1;
+                w2 = T(WENO_D[GP][2]) / k2;
+            } else {
+                w0 = T(WENO_D[GP][0]) * k0;
+                w1 = T(WENO_D[GP][1]) * k1;
+                w2 = T(WENO_D[GP][2]) * k2;
+            }
+        } else {
+            w0 = T(WENO_D[GP][0]) * (T(1) + k0);
+            w1 = T(WENO_D[GP][1]) * (T(1) + k1);
+            w2 = T(WENO_D[GP][2]) * (T(1) + k2);
+        }
+        const T sum = w0 + w1 + w2;
+        if constexpr (sizeof(T) == 8) {
+            w0 = w0 / sum; w1 = w1 / sum; w2 = w2 / sum;
+            return w0 * h0 + w1 * h1 + w2 * h2;
+        } else {
+            return (w0 * h0 + w1 * h1 + w2 * h2) * __builtin_amdgcn_rcpf(sum);
+        }
+    }
+};
+
+}  // namespace wfv

@@ -1,0 +1,49 @@
+// wenofv_geometry.h -- grid geometry and launch-shape defaults (host side).
+#pragma once
+
+namespace wfv {
+
+// Cell widths as CWENOFV::initializeSolver builds them for its `nranks` x-slabs
+// (src/WENOFV.cpp:90-123): x0 = xL + rank*(nx/nranks)*dx - 3*dx and a cell's width is
+// (x0 + (i+1)*dx) - (x0 + i*dx) in its rank's local index i.  These per-cell widths enter
+// the face quadrature and the RHS divisors (src/WENOFV.cpp:1774, 1793, 1809-1824); the
+// global dx, dy enter the CFL step and the limiter.  dxW[g] is the width of column g-1 as
+// column g's rank computes it (its ghost column when g starts a slab).
+inline void grid_widths(int nx, int ny, double xL, double xR, double yL, double yR, int nranks, double *dxC,
+                        double *dxW, double *dyC, double *xcen, double *ycen) {
+    const double dX = (xR - xL) / nx, dY = (yR - yL) / ny;
+    const int ng = 3;
+    if (nranks < 1 || nx % nranks) nranks = 1;
+    const int nloc = nx / nranks;
+    for (int g = 0; g < nx; ++g) {
+        const int rank = g / nloc;
+        const double procXLeft = xL + (double)(rank * nloc) * dX;
+        const double x0 = procXLeft - dX * ng;
+        const int ei = g - rank * nloc + ng;
+        const double xl = x0 + ei * dX, xr = x0 + (ei + 1) * dX;
+        if (dxC) dxC[g] = xr - xl;
+        if (dxW) dxW[g] = (x0 + ei * dX) - (x0 + (ei - 1) * dX);
+        if (xcen) xcen[g] = (xl + xr) / 2;
+    }
+    const double y0 = (yL + 0 * ny * dY) - dY * ng;
+    for (int j = 0; j < ny; ++j) {
+        const int ej = j + ng;
+        const double yl = y0 + ej * dY, yr = y0 + (ej + 1) * dY;
+        if (dyC) dyC[j] = yr - yl;
+        if (ycen) ycen[j] = (yl + yr) / 2;
+    }
+}
+
+// Rows per wavefront strip: long strips amortise the two recomputed halo rows, short ones
+// give more wavefronts; aim for >= 8 wavefronts per CU of a 256-CU device.
+inline int default_rows_per_strip(int nx, int ny) {
+    const int sx = (nx + 57) / 58;
+    const int cand[] = {64, 48, 32, 24, 16, 12, 8};
+    for (int ty : cand) {
+        const long waves = (long)sx * ((ny + ty - 1) / ty);
+        if (waves >= 2048) return ty;
+    }
+    return 8;
+}
+
+}  // namespace wfv

@@ -1,0 +1,64 @@
+// wenofv_kernels.h -- launch interface of the fused WENOFV stage kernels (library-internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "wenofv.h"
+#include "wenofv_device.h"
+
+namespace wfv {
+
+// One RK stage: Uo = (a*Un + b*U) + (c*dt)*rhs(U).  Un == nullptr stands for Un = U.
+struct StageArgs {
+    const double *U;      // stage input, SoA [4][ny][nx]
+    const double *Un;     // RK base state or nullptr
+    double *Uo;           // stage output (may alias Un, never U)
+    const double *dxC;    // [nx] column widths
+    const double *dxW;    // [nx] west-neighbour widths as the owning rank sees them
+    const double *dyC;    // [ny] row widths
+    const double *dt_ptr; // device dt (time loop on the device) or nullptr
+    const int *active;    // device loop flag or nullptr
+    double dt;            // host dt when dt_ptr == nullptr
+    double a, b, c;       // RK coefficients
+    Phys P;
+    double dX, dY, cfl, gravity;
+    int nx, ny, bc, flux, bp;
+    int ty;               // rows per strip
+    int strips_x, strips_y;
+    unsigned *flags;                 // error bits (WFV_FLAG_*)
+    unsigned long long *dtmin;       // if set: CFL minimum of the output state (bit pattern)
+    double *faces_out;               // parity hook: limited face values [cell][side][v][g]
+    double *theta_out;               // parity hook: limiter theta per cell
+};
+
+struct DtArgs {
+    const double *U;
+    Phys P;
+    double dX, dY, cfl;
+    int nx, ny;
+    unsigned *flags;
+    unsigned long long *dtmin;
+};
+
+// device-resident state of the reference time loop for one solver instance
+struct StepCtl {
+    unsigned long long dtmin_bits;  // CFL minimum of the current state (~0: none)
+    double dt;                      // dt of the step in progress
+    double now;                     // physical time
+    double t_end;
+    int count;                      // steps taken
+    int fixed_steps;                // > 0: unclipped fixed-step mode
+    int active;                     // 0 once the loop has finished
+    unsigned flags;                 // error bits
+    double *dts;                    // optional record of every dt (device memory)
+    int max_record;
+    int pad;
+};
+
+hipError_t launch_stage(const StageArgs &A, int precision, int recon, int scheme, hipStream_t s);
+hipError_t launch_dt(const DtArgs &A, hipStream_t s);
+hipError_t launch_step_control(StepCtl *S, hipStream_t s);
+int strip_width();
+
+}  // namespace wfv

@@ -1,0 +1,170 @@
+"""Python binding of libwenofv.so (the C ABI declared in include/wenofv.h).
+
+The binding mirrors the reference's CWENOFV surface (include/WENOFV.hpp:108-257): a solver
+is configured by the config/settings.cfg keys (TESTCASE, ELEMNUMX, ELEMNUMY, RKMETH, CFL,
+SCHEME, PPlimiter, RIEMANNFLUX, RECONSTRUCTIONTYPE) and exposes calculateDeltaT,
+performTimeIntegration, assembleRHS and the run loop.  State arrays are float64 numpy
+arrays of shape (4, ny, nx) holding (z1*rho1, z2*rho2, rho*u, rho*v).
+
+Loading fails loudly when the library has not been built: there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwenofv.so")
+
+OK, ERR_CONFIG, ERR_HIP, ERR_NONFINITE_DT, ERR_ARG = 0, 1, 2, 3, 4
+FLAG_NAN_FLUX, FLAG_BAD_THETA, FLAG_NAN_DT = 1, 2, 4
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("elem_num_x", ctypes.c_int), ("elem_num_y", ctypes.c_int), ("rk_method", ctypes.c_int),
+                ("cfl", ctypes.c_double), ("scheme", ctypes.c_int), ("pp_limiter", ctypes.c_int),
+                ("riemann_flux", ctypes.c_int), ("reconstruction", ctypes.c_int), ("precision", ctypes.c_int),
+                ("boundary", ctypes.c_int), ("geometry_ranks", ctypes.c_int), ("rows_per_strip", ctypes.c_int)]
+
+
+class Case(ctypes.Structure):
+    _fields_ = [("xL", ctypes.c_double), ("xR", ctypes.c_double), ("yL", ctypes.c_double), ("yR", ctypes.c_double),
+                ("rho10", ctypes.c_double), ("rho20", ctypes.c_double), ("sound1", ctypes.c_double),
+                ("sound2", ctypes.c_double), ("preb", ctypes.c_double), ("gravity", ctypes.c_double)]
+
+
+class WenofvError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libwenofv.so; raises if it is missing (build it with build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WenofvError(f"{LIB_PATH} not built: run python build.py in {HERE}")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.wfv_create.argtypes = [ctypes.POINTER(Options), ctypes.POINTER(Case), ctypes.POINTER(vp)]
+    L.wfv_destroy.argtypes = [vp]
+    L.wfv_last_error.restype = ctypes.c_char_p
+    L.wfv_last_error.argtypes = [vp]
+    for n in ("wfv_set_state", "wfv_get_state", "wfv_set_state_device", "wfv_get_state_device"):
+        getattr(L, n).argtypes = [vp, vp]
+    L.wfv_compute_dt.argtypes = [vp, _dp]
+    L.wfv_step.argtypes = [vp, ctypes.c_double]
+    L.wfv_assemble_rhs.argtypes = [vp, _dp, _dp, _dp]
+    L.wfv_advance.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                              _dp, _dp, ctypes.c_int]
+    L.wfv_get_flags.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+    L.wfv_synchronize.argtypes = [vp]
+    L.wfv_set_stream.argtypes = [vp, vp]
+    L.wfv_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp]
+    L.wfv_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Solver:
+    """One single-domain WENOFV solver instance on the current GPU."""
+
+    def __init__(self, nx, ny, domain=(0.0, 1.0, 0.0, 1.0), rho10=1.0, rho20=1.0, sound1=1.0, sound2=1.0, preb=0.0,
+                 gravity=0.0, scheme=0, recon=0, flux=0, bp=0, rk=2, cfl=0.4, precision=0, bc=0, geometry_ranks=4,
+                 rows_per_strip=0):
+        L = lib()
+        self.nx, self.ny = nx, ny
+        self.opt = Options(nx, ny, rk, cfl, scheme, bp, flux, recon, precision, bc, geometry_ranks, rows_per_strip)
+        xL, xR, yL, yR = domain
+        self.case = Case(xL, xR, yL, yR, rho10, rho20, sound1, sound2, preb, gravity)
+        h = ctypes.c_void_p()
+        rc = L.wfv_create(ctypes.byref(self.opt), ctypes.byref(self.case), ctypes.byref(h))
+        if rc != OK:
+            raise WenofvError(f"wfv_create failed ({rc}): {L.wfv_last_error(None).decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wfv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise WenofvError(f"wenofv error {rc}: {lib().wfv_last_error(self._h).decode()}")
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (4, self.ny, self.nx)
+        self._check(lib().wfv_set_state(self._h, U.ctypes.data))
+
+    def get_state(self):
+        U = np.empty((4, self.ny, self.nx))
+        self._check(lib().wfv_get_state(self._h, U.ctypes.data))
+        return U
+
+    def set_state_device(self, ptr):
+        self._check(lib().wfv_set_state_device(self._h, ctypes.c_void_p(ptr)))
+
+    def get_state_device(self, ptr):
+        self._check(lib().wfv_get_state_device(self._h, ctypes.c_void_p(ptr)))
+
+    def compute_dt(self):
+        dt = ctypes.c_double()
+        self._check(lib().wfv_compute_dt(self._h, ctypes.byref(dt)))
+        return dt.value
+
+    def step(self, dt):
+        self._check(lib().wfv_step(self._h, dt))
+
+    def assemble_rhs(self, want_faces=False):
+        rhs = np.empty((4, self.ny, self.nx))
+        faces = np.empty((self.ny, self.nx, 64)) if want_faces else None
+        theta = np.empty((self.ny, self.nx)) if want_faces else None
+        self._check(lib().wfv_assemble_rhs(self._h, _p(rhs), _p(faces), _p(theta)))
+        return rhs, faces, theta
+
+    def advance(self, t_end=0.0, fixed_steps=0, t_start=0.0, max_dts=0):
+        n = ctypes.c_int()
+        t = ctypes.c_double()
+        dts = np.zeros(max_dts) if max_dts else None
+        rc = lib().wfv_advance(self._h, t_start, t_end, fixed_steps, ctypes.byref(n), ctypes.byref(t), _p(dts),
+                               max_dts)
+        self._check(rc)
+        return n.value, t.value, (dts[: min(n.value, max_dts)] if max_dts else None)
+
+    def flags(self):
+        f = ctypes.c_uint()
+        self._check(lib().wfv_get_flags(self._h, ctypes.byref(f)))
+        return f.value
+
+    def set_stream(self, stream_handle):
+        """Run on a caller-owned HIP stream (e.g. torch.cuda.Stream().cuda_stream); 0 resets."""
+        self._check(lib().wfv_set_stream(self._h, ctypes.c_void_p(stream_handle or None)))
+
+    def synchronize(self):
+        self._check(lib().wfv_synchronize(self._h))
+
+    def grid(self):
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        dxC, dxW, dyC = np.empty(self.nx), np.empty(self.nx), np.empty(self.ny)
+        self._check(lib().wfv_grid(self._h, ctypes.byref(nx), ctypes.byref(ny), _p(dxC), _p(dxW), _p(dyC)))
+        return dict(dxC=dxC, dxW=dxW, dyC=dyC)
+
+
+def version():
+    return lib().wfv_version().decode()
