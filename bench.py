@@ -75,6 +75,8 @@ def main():
         sys.exit("no GPU visible")
     if a.workload == "liquid_column":
         wl = cases.liquid_column(precision=a.precision)
+    elif a.workload == "perturbed_column":
+        wl = cases.perturbed_column(precision=a.precision)
     else:
         sys.exit(f"unknown workload {a.workload}")
     sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs())
@@ -82,7 +84,7 @@ def main():
     stream = torch.cuda.Stream()
     sol.set_stream(stream.cuda_stream)
     if a.warmup:
-        sol.advance(fixed_steps=a.warmup)
+        sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
     sol.set_state(wl.U0)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

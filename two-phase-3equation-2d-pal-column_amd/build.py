@@ -22,7 +22,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-I" + os.path.join(REPO, "include"), "-I" + CSRC, "-Wno-unused-command-line-argument"]
 # translation units and their contraction policy
 UNITS = [
-    ("wenofv_kernels.hip", ["-ffp-contract=off"]),
+    ("wenofv_kernels_fp64.hip", ["-ffp-contract=off"]),
+    ("wenofv_kernels_mixed.hip", ["-ffp-contract=fast"]),
+    ("wenofv_control.hip", ["-ffp-contract=off"]),
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),
 ]
 
@@ -31,7 +33,7 @@ def build(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for src, extra in UNITS:
+    for src, extra in UNITS:  # compiled one after another (each unit takes ~10-30 s)
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:

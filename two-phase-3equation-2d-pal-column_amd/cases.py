@@ -125,8 +125,12 @@ def perturbed_column(nx=4096, ny=2048, precision=1, seed=3):
     s = sum(a[m] * np.sin((m + 1) * th + ph[m]) for m in range(16))
     a *= 0.01 / np.abs(s).max()
 
+    # radius on a fine periodic angle table (linear interpolation error < 1e-9)
+    tab_t = np.linspace(-np.pi, np.pi, 1 << 16)
+    tab_r = 0.15 * (1 + sum(a[m] * np.sin((m + 1) * tab_t + ph[m]) for m in range(16)))
+
     def radius(t):
-        return 0.15 * (1 + sum(a[m] * np.sin((m + 1) * t + ph[m]) for m in range(16)))
+        return np.interp(t, tab_t, tab_r)
 
     dom = (0.0, 2.0, 0.0, 1.0)
     U0 = cell_average(nx, ny, dom, _column_state(prm, 0.5, 0.5, radius, 0.2, 2.0))

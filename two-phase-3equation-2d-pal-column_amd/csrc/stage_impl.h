@@ -1,4 +1,5 @@
-// wenofv_kernels.hip -- fused WENOFV RK stage for gfx950.
+// stage_impl.h -- fused WENOFV RK stage for gfx950 (template; instantiated per precision
+// policy by wenofv_kernels_fp64.hip and wenofv_kernels_mixed.hip).
 //
 // One launch performs a whole SSP-RK stage of the reference's assembleRHS + RK update
 // (src/WENOFV.cpp:436-471, 568-615): ghost fill, y- and x-direction WENO reconstruction
@@ -21,6 +22,7 @@
 //   5. completes the RHS of the cell of row r-1, applies the RK combination and stores.
 // Lanes 3..60 own output cells; lanes 2 and 61 recompute the neighbouring strips'
 // edge cells, whose face values the strip's boundary fluxes need.
+#pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -106,16 +108,20 @@ WFV_DEV void recon4(const W s[5], W out[4]) {
     out[3] = st.template node<3>();
 }
 
-// LDS per wavefront
+// LDS per wavefront.  The y-sweep values are dead once the x-sweep has read them, so the
+// west face values of the same row reuse their space.
 template <class PR>
 struct WaveLds {
-    typename PR::W y[16][LANES];  // y-sweep values [g*4+v][lane]
-    typename PR::F west[16][LANES];  // limited west face values [v*4+g][lane]
-    double fx[4][LANES];          // integrated flux through the east face [v][lane]
+    union {
+        typename PR::W y[16][LANES];     // y-sweep values [g*4+v][lane]
+        typename PR::F west[16][LANES];  // limited west face values [v*4+g][lane]
+    };
+    double fx[4][LANES];  // integrated flux through the east face [v][lane]
 };
 
-template <class PR, int RECON, int SCHEME, int WAVES>
-__global__ __launch_bounds__(LANES *WAVES) void stage_kernel(StageArgs A) {
+template <class PR, int RECON, int SCHEME, int WAVES, int MINW>
+__global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(
+    StageArgs A) {
     using W = typename PR::W;
     using F = typename PR::F;
     __shared__ WaveLds<PR> lds_all[WAVES];
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(LANES *WAVES) void stage_kernel(StageArgs A) {
                 // getPositivityPreservingTheta (src/WENOFV.cpp:1936-2000), double in both policies
                 const double *Uc = win[2];
                 Eos e;
-                eos(P, Uc, e);
+                eos_p<!PR::exact>(P, Uc, e);
                 const double u = Uc[2] / (Uc[0] + Uc[1]);
                 const double v = Uc[3] / (Uc[0] + Uc[1]);
                 const double a1 = fabs(u * 1.0 + v * 0.0) + e.c;
@@ -407,6 +413,7 @@ __global__ __launch_bounds__(LANES *WAVES) void stage_kernel(StageArgs A) {
             if (interior_row) {
                 // ---- 4. face east of cell (xc, r): right face | neighbour lane's left face
                 //      (src/WENOFV.cpp:1707-1725, 1764-1778)
+                wave_sync();  // every lane has read the y-sweep values the west faces overwrite
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(LANES *WAVES) void stage_kernel(StageArgs A) {
                 if (A.dtmin) {
                     // calculateDeltaT on the new state (src/WENOFV.cpp:489-513)
                     Eos e;
-                    eos(P, Unew, e);
+                    eos_p<!PR::exact>(P, Unew, e);
                     const double u = Unew[2] / (Unew[0] + Unew[1]);
                     const double v = Unew[3] / (Unew[0] + Unew[1]);
                     const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
@@ -499,104 +506,5 @@ __global__ __launch_bounds__(LANES *WAVES) void stage_kernel(StageArgs A) {
         if (A.dtmin && dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
     }
 }
-
-// CFL time step of a state (calculateDeltaT, src/WENOFV.cpp:474-540); positive doubles
-// order like their bit patterns, so the minimum is an integer atomic.
-__global__ __launch_bounds__(256) void dt_kernel(DtArgs A) {
-    const size_t nc = (size_t)A.nx * A.ny;
-    double dtloc = 1.0e300;
-    unsigned err = 0;
-    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (size_t)gridDim.x * blockDim.x) {
-        const double U[4] = {A.U[c], A.U[nc + c], A.U[2 * nc + c], A.U[3 * nc + c]};
-        Eos e;
-        eos(A.P, U, e);
-        const double u = U[2] / (U[0] + U[1]);
-        const double v = U[3] / (U[0] + U[1]);
-        const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
-        const double ey = fabs(u * 0.0 + v * 1.0) + e.c;
-        const double tmp = A.cfl / (ex / A.dX + ey / A.dY);
-        if (isnan(tmp)) err |= WFV_FLAG_NAN_DT;
-        else dtloc = fmin(dtloc, tmp);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        err |= __shfl_xor(err, off, LANES);
-        dtloc = fmin(dtloc, __shfl_xor(dtloc, off, LANES));
-    }
-    if ((threadIdx.x & (LANES - 1)) == 0) {
-        if (err && A.flags) atomicOr(A.flags, err);
-        if (dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
-    }
-}
-
-// Device-side step control of the reference time loop (src/WENOFV.cpp:318-332):
-// dt = min(timestep, T - now) with timestep = min(1.0, CFL minimum); the loop stops when
-// |now - T| <= 1e-10, or after a fixed number of steps.  It writes the dt the following
-// stages read, advances `now`, and re-arms the CFL minimum for the last stage.
-__global__ void step_control_kernel(StepCtl *S) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (S->active == 0) return;
-    bool go;
-    if (S->fixed_steps > 0) go = S->count < S->fixed_steps;
-    else go = fabs(S->now - S->t_end) > 1e-10;
-    if (!go || (S->flags & WFV_FLAG_NAN_DT)) {
-        S->active = 0;
-        return;
-    }
-    double timestep = 1.0;
-    const double m = __longlong_as_double((long long)S->dtmin_bits);
-    if (S->dtmin_bits != ~0ull && !(m > timestep)) timestep = m;
-    if (S->fixed_steps <= 0) timestep = stdMin(timestep, S->t_end - S->now);
-    S->dt = timestep;
-    if (S->count < S->max_record && S->dts) S->dts[S->count] = timestep;
-    S->now += timestep;
-    S->count += 1;
-    S->dtmin_bits = ~0ull;
-}
-
-// ---------------------------------------------------------------------- host launchers
-template <class PR, int RECON, int SCHEME>
-static hipError_t launch_t(const StageArgs &A, hipStream_t s) {
-    constexpr int WAVES = 4;
-    const int nstrip = A.strips_x * A.strips_y;
-    const int blocks = (nstrip + WAVES - 1) / WAVES;
-    hipLaunchKernelGGL((stage_kernel<PR, RECON, SCHEME, WAVES>), dim3(blocks), dim3(LANES * WAVES), 0, s, A);
-    return hipGetLastError();
-}
-
-template <class PR>
-static hipError_t launch_p(const StageArgs &A, int recon, int scheme, hipStream_t s) {
-    switch (recon * 2 + scheme) {
-    case 0: return launch_t<PR, 0, 0>(A, s);
-    case 1: return launch_t<PR, 0, 1>(A, s);
-    case 2: return launch_t<PR, 1, 0>(A, s);
-    case 3: return launch_t<PR, 1, 1>(A, s);
-    case 4: return launch_t<PR, 2, 0>(A, s);
-    case 5: return launch_t<PR, 2, 1>(A, s);
-    case 6: return launch_t<PR, 3, 0>(A, s);
-    case 7: return launch_t<PR, 3, 1>(A, s);
-    }
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_stage(const StageArgs &A, int precision, int recon, int scheme, hipStream_t s) {
-    if (precision == 1) return launch_p<PrecMixed>(A, recon, scheme, s);
-    return launch_p<PrecFP64>(A, recon, scheme, s);
-}
-
-hipError_t launch_dt(const DtArgs &A, hipStream_t s) {
-    const size_t nc = (size_t)A.nx * A.ny;
-    int blocks = (int)((nc + 255) / 256);
-    if (blocks > 4096) blocks = 4096;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(dt_kernel, dim3(blocks), dim3(256), 0, s, A);
-    return hipGetLastError();
-}
-
-hipError_t launch_step_control(StepCtl *S, hipStream_t s) {
-    hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(64), 0, s, S);
-    return hipGetLastError();
-}
-
-int strip_width() { return STRIP_W; }
 
 }  // namespace wfv

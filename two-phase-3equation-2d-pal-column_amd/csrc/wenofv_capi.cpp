@@ -326,15 +326,16 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
     // CFL minimum of the initial state
     DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags, &c->d_ctl->dtmin_bits};
     HIPCHK(c, launch_dt(D, c->stream));
-    // a batch of steps captured once and replayed; steps past the end of the loop exit at once
-    const int batch = fixed_steps > 0 ? (fixed_steps < 32 ? fixed_steps : 32) : 32;
-    if (!c->graph || c->graph_steps != batch) {
-        if (c->graph) hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
+    // Steps are replayed from a graph of GRAPH_STEPS steps captured once per context; a
+    // step enqueued after the loop has finished returns at once (device flag `active`).
+    // A fixed-step run enqueues its remainder as individual launches, so it never runs past
+    // its last step.
+    constexpr int GRAPH_STEPS = 32;
+    if (!c->graph) {
         hipGraph_t g;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = WFV_OK;
-        for (int k = 0; k < batch && rc == WFV_OK; ++k) {
+        for (int k = 0; k < GRAPH_STEPS && rc == WFV_OK; ++k) {
             if (launch_step_control(c->d_ctl, c->stream) != hipSuccess) rc = WFV_ERR_HIP;
             if (rc == WFV_OK) rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true);
         }
@@ -344,13 +345,26 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
         e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(c, WFV_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
-        c->graph_steps = batch;
+        c->graph_steps = GRAPH_STEPS;
     }
-    for (;;) {
-        HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    if (fixed_steps > 0) {
+        const int full = fixed_steps / GRAPH_STEPS, rest = fixed_steps % GRAPH_STEPS;
+        for (int k = 0; k < full; ++k) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+        for (int k = 0; k < rest; ++k) {
+            HIPCHK(c, launch_step_control(c->d_ctl, c->stream));
+            int rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true);
+            if (rc) return rc;
+        }
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
-        if (!h.active) break;
+    } else {
+        for (;;) {
+            // a few graphs in flight between host checks of the loop flag
+            for (int k = 0; k < 4; ++k) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+            if (!h.active) break;
+        }
     }
     if (nsteps) *nsteps = h.count;
     if (t_out) *t_out = h.now;

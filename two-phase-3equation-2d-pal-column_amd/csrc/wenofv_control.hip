@@ -1,0 +1,100 @@
+// wenofv_control.hip -- CFL reduction and device-side step control of the time loop.
+#include <hip/hip_runtime.h>
+
+#include "wenofv_device.h"
+#include "wenofv_kernels.h"
+
+namespace wfv {
+
+constexpr int DT_THREADS = 256;
+
+// CFL time step of a state (calculateDeltaT, src/WENOFV.cpp:474-540).  Positive doubles
+// order like their bit patterns, so the minimum is an integer atomic; one atomic per block.
+__global__ __launch_bounds__(DT_THREADS) void dt_kernel(DtArgs A) {
+    __shared__ double smin[DT_THREADS / 64];
+    __shared__ unsigned serr[DT_THREADS / 64];
+    const size_t nc = (size_t)A.nx * A.ny;
+    double dtloc = 1.0e300;
+    unsigned err = 0;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (size_t)gridDim.x * blockDim.x) {
+        const double U[4] = {A.U[c], A.U[nc + c], A.U[2 * nc + c], A.U[3 * nc + c]};
+        Eos e;
+        eos(A.P, U, e);
+        const double u = U[2] / (U[0] + U[1]);
+        const double v = U[3] / (U[0] + U[1]);
+        const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
+        const double ey = fabs(u * 0.0 + v * 1.0) + e.c;
+        const double tmp = A.cfl / (ex / A.dX + ey / A.dY);
+        if (isnan(tmp)) err |= WFV_FLAG_NAN_DT;
+        else dtloc = fmin(dtloc, tmp);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        err |= __shfl_xor(err, off, 64);
+        dtloc = fmin(dtloc, __shfl_xor(dtloc, off, 64));
+    }
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        smin[w] = dtloc;
+        serr[w] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < DT_THREADS / 64; ++k) {
+            dtloc = fmin(dtloc, smin[k]);
+            err |= serr[k];
+        }
+        if (err && A.flags) atomicOr(A.flags, err);
+        if (dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
+    }
+}
+
+// Device-side step control of the reference time loop (src/WENOFV.cpp:318-332):
+// dt = min(timestep, T - now) with timestep = min(1.0, CFL minimum); the loop stops when
+// |now - T| <= 1e-10, or after a fixed number of steps.  It writes the dt the following
+// stages read, advances `now`, and re-arms the CFL minimum for the last stage.
+__global__ void step_control_kernel(StepCtl *S) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (S->active == 0) return;
+    bool go;
+    if (S->fixed_steps > 0) go = S->count < S->fixed_steps;
+    else go = fabs(S->now - S->t_end) > 1e-10;
+    if (!go || (S->flags & WFV_FLAG_NAN_DT)) {
+        S->active = 0;
+        return;
+    }
+    double timestep = 1.0;
+    const double m = __longlong_as_double((long long)S->dtmin_bits);
+    if (S->dtmin_bits != ~0ull && !(m > timestep)) timestep = m;
+    if (S->fixed_steps <= 0) timestep = stdMin(timestep, S->t_end - S->now);
+    S->dt = timestep;
+    if (S->count < S->max_record && S->dts) S->dts[S->count] = timestep;
+    S->now += timestep;
+    S->count += 1;
+    S->dtmin_bits = ~0ull;
+}
+
+hipError_t launch_stage_fp64(const StageArgs &A, int recon, int scheme, hipStream_t s);
+hipError_t launch_stage_mixed(const StageArgs &A, int recon, int scheme, hipStream_t s);
+
+hipError_t launch_stage(const StageArgs &A, int precision, int recon, int scheme, hipStream_t s) {
+    if (precision == 1) return launch_stage_mixed(A, recon, scheme, s);
+    return launch_stage_fp64(A, recon, scheme, s);
+}
+
+hipError_t launch_dt(const DtArgs &A, hipStream_t s) {
+    const size_t nc = (size_t)A.nx * A.ny;
+    int blocks = (int)((nc + DT_THREADS - 1) / DT_THREADS);
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(dt_kernel, dim3(blocks), dim3(DT_THREADS), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_control(StepCtl *S, hipStream_t s) {
+    hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(64), 0, s, S);
+    return hipGetLastError();
+}
+
+int strip_width() { return 58; }
+
+}  // namespace wfv

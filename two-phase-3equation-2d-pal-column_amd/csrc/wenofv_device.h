@@ -76,6 +76,44 @@ WFV_DEV void eos(const Phys &P, const double U[4], Eos &e) {
     e.c = 1.0 / sc;
 }
 
+// Double reciprocal and inverse square root from the hardware estimates plus one
+// Newton-Raphson step (relative error ~1e-15); used only by the mixed policy.
+WFV_DEV double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+WFV_DEV double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x * y;
+    return fma(y, fma(-h, y, 0.5), y);
+}
+
+// The same equation of state with the algebra rearranged to one square root and three
+// reciprocals (mixed policy; still double): with N = dq + S and D = 2 z2rho2 c2^2,
+// z1 = N/(D+N), z2 = D/(D+N), rho2 = (D+N)/(2 c2^2), rho1 = z1rho1 (D+N)/N, and
+// 1/c^2 = rho (z1^2/(z1rho1 c1^2) + z2^2/(z2rho2 c2^2)).
+WFV_DEV void eos_fast(const Phys &P, const double U[4], Eos &e) {
+    const double a = U[0], b = U[1];
+    const double dq = P.q - (b * P.c2_2 - a * P.c1_2);
+    const double S = sqrt(dq * dq + 4.0 * a * P.c1_2 * b * P.c2_2);
+    const double N = dq + S, D = 2.0 * b * P.c2_2;
+    const double inv = rcp_nr(D + N);
+    const double z1 = N * inv, z2 = D * inv;
+    e.rho = a + b;
+    e.z1 = z1;
+    const double rho2 = (D + N) * (0.5 / P.c2_2);
+    const double rho1 = a * (D + N) * rcp_nr(N);
+    e.p = (z1 < z2) ? P.c2_2 * (rho2 - P.rho20) : P.c1_2 * (rho1 - P.rho10);
+    const double ic2 = e.rho * (z1 * z1 * rcp_nr(a * P.c1_2) + z2 * z2 * rcp_nr(b * P.c2_2));
+    e.c = rsq_nr(ic2);
+}
+
+template <bool FAST>
+WFV_DEV void eos_p(const Phys &P, const double U[4], Eos &e) {
+    if constexpr (FAST) eos_fast(P, U, e);
+    else eos(P, U, e);
+}
+
 // ------------------------------------------------------------------ Riemann fluxes
 // F = face-normal flux through (NX, NY) in {(1,0), (0,1)}.  The literal products with
 // the normal's components are kept so that the double policy matches the reference's
@@ -96,8 +134,8 @@ WFV_DEV void riemann_llf(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
     double dl[4] = {(double)UL[0], (double)UL[1], (double)UL[2], (double)UL[3]};
     double dr[4] = {(double)UR[0], (double)UR[1], (double)UR[2], (double)UR[3]};
     Eos el, er;
-    eos(P, dl, el);
-    eos(P, dr, er);
+    eos_p<sizeof(T) == 4>(P, dl, el);
+    eos_p<sizeof(T) == 4>(P, dr, er);
     const T nx = T(NX), ny = T(NY);
     const T rl = T(el.rho), rr = T(er.rho);
     const T ul = dv(UL[2], rl), vl = dv(UL[3], rl);
@@ -118,8 +156,8 @@ WFV_DEV void riemann_hllc(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
     double dl[4] = {(double)UL[0], (double)UL[1], (double)UL[2], (double)UL[3]};
     double dr[4] = {(double)UR[0], (double)UR[1], (double)UR[2], (double)UR[3]};
     Eos el, er;
-    eos(P, dl, el);
-    eos(P, dr, er);
+    eos_p<sizeof(T) == 4>(P, dl, el);
+    eos_p<sizeof(T) == 4>(P, dr, er);
     const T nx = T(NX), ny = T(NY);
     const T rho_L = T(el.rho), rho_R = T(er.rho);
     const T u_L = dv(UL[2], rho_L), v_L = dv(UL[3], rho_L);
@@ -177,7 +215,7 @@ WFV_DEV void riemann(const Phys &P, int type, const T UL[4], const T UR[4], T F[
 template <class T, bool XDIR>
 WFV_DEV void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
     Eos e;
-    eos(P, U, e);
+    eos_p<sizeof(T) == 4>(P, U, e);
     const double rho = e.rho, u = U[2] / rho, v = U[3] / rho, c = e.c;
     const double z1 = e.z1, z2 = 1.0 - z1;
     const double zrho1 = U[0] * P.c1_2 / (z1 * z1);
