@@ -2,11 +2,18 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
-A step is one SSP-RK3 time step (3 fused stage launches) over the workload; inputs are
-resident in HBM before the timed region, which is bracketed by device synchronisation and
-timed with HIP events recorded on the solver's stream.  Prints one JSON line.
+A step is one SSP-RK3 time step (3 fused stage launches) over the whole workload; inputs
+are resident in HBM before the timed region, which is bracketed by device synchronisation
+and timed with HIP events recorded on the solver's stream.  Prints one JSON line.
 
-Workloads: liquid_column (1024x512, walls, WENO-Z + HLLC + limiter, FP64 or mixed).
+Workloads
+  batch256 (default)  256 independent liquid-column instances, grids drawn from
+                      {64,128,256,512}^2 (seed 4), mixed precision, WENO-Z + HLLC +
+                      limiter, RK3, CFL 0.4, one persistent launch per stage over all
+                      instances.  The same batch in all-FP64 gives the baseline throughput
+                      and the reference state for the error check (relative L1 per field).
+  liquid_column       1024x512 single case (walls, WENO-Z + HLLC + limiter), --precision
+  perturbed_column    4096x2048 single case, --precision
 """
 import argparse
 import json
@@ -20,6 +27,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(REPO, "two-phase-3equation-2d-pal-column_amd")
 sys.path.insert(0, PKG)
 
+METRIC = "cell-stage-updates/s mixed+batched vs FP64 GPU at ≤1e-5 L1; achieved HBM GB/s"
 BYTES_PER_CELL_STAGE = 256.0 / 3.0  # FP64 state, 4 fields, SSP-RK3: (64 + 96 + 96) B per cell-step
 HBM_PEAK_GBS = 8000.0
 
@@ -27,26 +35,55 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="liquid_column")
-    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0: the workload's own count)")
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--workload", default="batch256")
+    ap.add_argument("--precision", type=int, default=1)
+    ap.add_argument("--ty", type=int, default=0, help="rows per wavefront strip (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-fp64", action="store_true", help="batch256: skip the FP64 baseline run")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
 
-def cpu_baseline(wl, seconds):
+def derived(U, rho10, rho20, sound1, sound2, **_):
+    c1, c2 = sound1 ** 2, sound2 ** 2
+    q = rho20 * c2 - rho10 * c1
+    qs = U[1] * c2 - U[0] * c1
+    r = ((q - qs) + np.sqrt((q - qs) ** 2 + 4.0 * U[0] * c1 * U[1] * c2)) / (2.0 * U[1] * c2)
+    z1 = r / (1 + r)
+    rho = U[0] + U[1]
+    return z1, rho, U[2] / rho, U[3] / rho
+
+
+def field_errors(Um, Ud, params):
+    """Relative L1 of the mixed state against the FP64 state: conserved fields, z1, rho, u,
+    v.  Momentum and velocity components are normalised by their vector's magnitude."""
+    z1m, rm, um, vm = derived(Um, **params)
+    z1d, rd, ud, vd = derived(Ud, **params)
+    mom = np.abs(np.hypot(Ud[2], Ud[3])).sum()
+    vel = np.abs(np.hypot(ud, vd)).sum()
+
+    def l1(a, b, s=None):
+        return float(np.abs(a - b).sum() / max(np.abs(b).sum() if s is None else s, 1e-300))
+
+    errs = {"z1rho1": l1(Um[0], Ud[0]), "z2rho2": l1(Um[1], Ud[1]), "rhou": l1(Um[2], Ud[2], mom),
+            "rhov": l1(Um[3], Ud[3], mom), "z1": l1(z1m, z1d), "rho": l1(rm, rd), "u": l1(um, ud, vel),
+            "v": l1(vm, vd, vel)}
+    linf = {"z1": float(np.abs(z1m - z1d).max()), "rho": float(np.abs(rm - rd).max() / np.abs(rd).max()),
+            "u": float(np.abs(um - ud).max() / np.hypot(ud, vd).max()),
+            "v": float(np.abs(vm - vd).max() / np.hypot(ud, vd).max())}
+    return errs, linf, (float(z1m.min()), float(z1m.max()))
+
+
+def cpu_baseline_oracle(nx, ny, domain, bc, params, opts, U0, seconds, label):
     """The oracle (FP64 CPU restatement, OpenMP over rows) on the same initial state, for as
-    many steps as fit in `seconds` (at least one)."""
+    many RK3 steps as fit in `seconds` (at least one)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
-    kw = dict(wl.params)
-    o = orc.Oracle(wl.nx, wl.ny, domain=wl.domain, bc=wl.bc, nranks=1, scheme=wl.opts["scheme"],
-                   recon=wl.opts["recon"], flux=wl.opts["flux"], bp=wl.opts["bp"], rk=wl.opts["rk"],
-                   cfl=wl.opts["cfl"], **kw)
-    U = wl.U0
-    steps = 0
+    o = orc.Oracle(nx, ny, domain=domain, bc=bc, nranks=1, scheme=opts["scheme"], recon=opts["recon"],
+                   flux=opts["flux"], bp=opts["bp"], rk=opts["rk"], cfl=opts["cfl"], **params)
+    U, steps = U0, 0
     t0 = time.perf_counter()
     while True:
         U, dts, t, err = o.run(U, fixed_steps=1)
@@ -55,10 +92,143 @@ def cpu_baseline(wl, seconds):
             break
     el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    rate = wl.nx * wl.ny * 3 * steps / el
-    return {"value": rate, "unit": "cell-stage-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{wl.name} {wl.nx}x{wl.ny}, {steps} RK3 steps from the initial state, FP64 oracle "
-                      f"(OpenMP, {cores} threads), {el:.1f} s"}
+    return {"value": nx * ny * 3 * steps / el, "unit": "cell-stage-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{label}: {nx}x{ny}, {steps} RK3 steps from the initial state, all-FP64 oracle "
+                      f"(OpenMP {cores} threads), {el:.1f} s"}
+
+
+def timed(torch, obj, stream, steps):
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    res = obj.advance(fixed_steps=steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1), res
+
+
+def roofline(cells, stage_ms):
+    achieved = BYTES_PER_CELL_STAGE * cells / (stage_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "note": "VALU-bound stencil: achieved = 85.33 algorithmic B per cell-stage x cells / mean stage time"}
+
+
+def run_batch(a, torch, wenofv, cases):
+    specs = cases.batch_specs(256, seed=4)
+    wls = [cases.batch_instance(sp) for sp in specs]
+    opts = dict(wls[0].opts)
+    steps = a.steps or wls[0].steps
+    insts = [dict(nx=w.nx, ny=w.ny, bc=w.bc, domain=w.domain, **w.params) for w in wls]
+    cells = sum(w.nx * w.ny for w in wls)
+    stream = torch.cuda.Stream()
+    rng = np.random.default_rng(13)
+    results = {}
+    # runs: mixed (timed), all-FP64 (timed; the baseline and the reference state), and
+    # all-FP64 from initial data perturbed by 1e-13 (the FP64 solution's own sensitivity)
+    runs = [("mixed", 1, False)] + ([] if a.no_fp64 else [("fp64", 0, False), ("fp64_perturbed", 0, True)])
+    for tag, prec, perturb in runs:
+        b = wenofv.Batch(insts, rows_per_strip=a.ty, **dict(opts, precision=prec))
+        b.set_stream(stream.cuda_stream)
+        if a.warmup:
+            for k, w in enumerate(wls):
+                b.set_state(k, w.U0)
+            b.advance(fixed_steps=max(a.warmup, 16))  # also captures the step graph
+        for k, w in enumerate(wls):
+            b.set_state(k, w.U0 * (1 + 1e-13 * rng.standard_normal(w.U0.shape)) if perturb else w.U0)
+        ms, (n, t, f) = timed(torch, b, stream, steps)
+        assert (n == steps).all(), n
+        results[tag] = dict(ms=ms, flags=int(np.bitwise_or.reduce(f)), states=[b.get_state(k) for k in range(len(wls))])
+        b.close()
+    rate = cells * 3 * steps / (results["mixed"]["ms"] * 1e-3)
+    out = {
+        "metric": METRIC, "value": rate, "unit": "cell-stage-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": results["mixed"]["ms"] / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded initial conditions, no dataset)",
+        "config": {"workload": f"batch256: 256 liquid-column instances (seed 4), grids {{64,128,256,512}}^2, walls, "
+                               f"mixed WENO-Z HLLC BP RK3 CFL0.4, {steps} steps each, one launch per stage",
+                   "instances": len(wls), "cells": cells, "flags": results["mixed"]["flags"]},
+        "roofline": roofline(cells, results["mixed"]["ms"] / (3 * steps)),
+    }
+    if "fp64" in results:
+        fp64_rate = cells * 3 * steps / (results["fp64"]["ms"] * 1e-3)
+        keys = ("z1rho1", "z2rho2", "rhou", "rhov", "z1", "rho", "u", "v")
+        num = {k: 0.0 for k in keys}
+        den = {k: 0.0 for k in keys}
+        inst_err, inst_spread, zb, mass = [], [], [1.0, 0.0], 0.0
+        for k, w in enumerate(wls):
+            Um, Ud, Up = (results[t]["states"][k] for t in ("mixed", "fp64", "fp64_perturbed"))
+            z1m, rm, um, vm = derived(Um, **w.params)
+            z1d, rd, ud, vd = derived(Ud, **w.params)
+            mom, vel = np.hypot(Ud[2], Ud[3]).sum(), np.hypot(ud, vd).sum()
+            pairs = {"z1rho1": (Um[0], Ud[0], None), "z2rho2": (Um[1], Ud[1], None), "rhou": (Um[2], Ud[2], mom),
+                     "rhov": (Um[3], Ud[3], mom), "z1": (z1m, z1d, None), "rho": (rm, rd, None),
+                     "u": (um, ud, vel), "v": (vm, vd, vel)}
+            for key, (x, y, sc) in pairs.items():
+                num[key] += np.abs(x - y).sum()
+                den[key] += np.abs(y).sum() if sc is None else sc
+            inst_err.append(max(field_errors(Um, Ud, w.params)[0].values()))
+            inst_spread.append(max(field_errors(Up, Ud, w.params)[0].values()))
+            zb = [min(zb[0], float(z1m.min())), max(zb[1], float(z1m.max()))]
+            m0 = w.U0[:2].sum(axis=(1, 2))
+            mass = max(mass, float(np.abs(Um[:2].sum(axis=(1, 2)) / m0 - 1).max()))
+        agg = {k: float(num[k] / den[k]) for k in keys}
+        inst_err, inst_spread = np.array(inst_err), np.array(inst_spread)
+        out["fp64_gpu"] = {"value": fp64_rate, "ms_per_step": results["fp64"]["ms"] / steps,
+                           "flags": results["fp64"]["flags"], "note": "the same batch and launches in all-FP64"}
+        out["speedup_vs_fp64_gpu"] = rate / fp64_rate
+        out["accuracy"] = {
+            "rel_l1_per_field_batch": agg, "max_rel_l1_batch": max(agg.values()), "tolerance": 1e-5,
+            "batch_within_tolerance": max(agg.values()) <= 1e-5,
+            "per_instance_max_field_rel_l1": {"median": float(np.median(inst_err)),
+                                              "p90": float(np.percentile(inst_err, 90)), "max": float(inst_err.max()),
+                                              "fraction_within_tolerance": float((inst_err <= 1e-5).mean())},
+            "fp64_self_spread_1e-13_perturbation": {"median": float(np.median(inst_spread)),
+                                                    "p90": float(np.percentile(inst_spread, 90)),
+                                                    "max": float(inst_spread.max())},
+            "z1_range": zb, "z1_in_unit_interval": zb[0] >= 0.0 and zb[1] <= 1.0, "max_phase_mass_drift": mass}
+    if not a.no_cpu:
+        w = next(w for w in wls if (w.nx, w.ny) == (256, 256))
+        out["cpu_baseline"] = cpu_baseline_oracle(w.nx, w.ny, w.domain, w.bc, w.params, opts, w.U0, a.cpu_seconds,
+                                                  "one 256x256 instance of the batch")
+    return out
+
+
+def run_single(a, torch, wenofv, cases):
+    if a.workload == "liquid_column":
+        wl = cases.liquid_column(precision=a.precision)
+    elif a.workload == "perturbed_column":
+        wl = cases.perturbed_column(precision=a.precision)
+    else:
+        sys.exit(f"unknown workload {a.workload}")
+    steps = a.steps or 200
+    sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty))
+    stream = torch.cuda.Stream()
+    sol.set_stream(stream.cuda_stream)
+    sol.set_state(wl.U0)
+    if a.warmup:
+        sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
+    sol.set_state(wl.U0)
+    ms, (n, t, _) = timed(torch, sol, stream, steps)
+    assert n == steps, (n, steps)
+    cells = wl.nx * wl.ny
+    U = sol.get_state()
+    prec = wl.opts.get("precision", 0)
+    out = {
+        "metric": METRIC, "value": cells * 3 * steps / (ms * 1e-3), "unit": "cell-stage-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32+f64",
+        "data": "synthetic (seeded initial condition, no dataset)",
+        "config": {"workload": f"{wl.name} {wl.nx}x{wl.ny} walls WENO-Z HLLC BP RK3 CFL0.4",
+                   "precision": "fp64" if prec == 0 else "mixed", "cells": cells, "final_time": t,
+                   "flags": sol.flags(), "finite": bool(np.isfinite(U).all())},
+        "roofline": roofline(cells, ms / (3 * steps)),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_oracle(wl.nx, wl.ny, wl.domain, wl.bc, wl.params, wl.opts, wl.U0,
+                                                  a.cpu_seconds, wl.name)
+    return out
 
 
 def main():
@@ -73,57 +243,7 @@ def main():
 
     if not torch.cuda.is_available():
         sys.exit("no GPU visible")
-    if a.workload == "liquid_column":
-        wl = cases.liquid_column(precision=a.precision)
-    elif a.workload == "perturbed_column":
-        wl = cases.perturbed_column(precision=a.precision)
-    else:
-        sys.exit(f"unknown workload {a.workload}")
-    sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs())
-    sol.set_state(wl.U0)
-    stream = torch.cuda.Stream()
-    sol.set_stream(stream.cuda_stream)
-    if a.warmup:
-        sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
-    sol.set_state(wl.U0)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    n, t, _ = sol.advance(fixed_steps=a.steps)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    flags = sol.flags()
-    assert n == a.steps, (n, a.steps)
-    cells = wl.nx * wl.ny
-    stages = 3 * a.steps
-    rate = cells * stages / (ms * 1e-3)
-    stage_ms = ms / stages
-    achieved = BYTES_PER_CELL_STAGE * cells / (stage_ms * 1e-3) / 1e9
-    U = sol.get_state()
-    out = {
-        "metric": "cell-stage-updates/s",
-        "value": rate,
-        "unit": "cell-stage-updates/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms / a.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if wl.opts.get("precision", 0) == 0 else "f32+f64",
-        "data": "synthetic (seeded initial condition, no dataset)",
-        "config": {"workload": f"{wl.name} {wl.nx}x{wl.ny} walls WENO-Z HLLC BP RK3 CFL0.4",
-                   "precision": "fp64" if wl.opts.get("precision", 0) == 0 else "mixed",
-                   "cells": cells, "final_time": t, "flags": flags,
-                   "finite": bool(np.isfinite(U).all())},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "note": "VALU-bound stencil; achieved = 85.33 B/cell-stage x cells / mean stage time"},
-    }
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+    out = run_batch(a, torch, wenofv, cases) if a.workload == "batch256" else run_single(a, torch, wenofv, cases)
     print(json.dumps(out), flush=True)
 
 

@@ -104,6 +104,37 @@ int wfv_synchronize(wfv_ctx *ctx);
    device). */
 int wfv_set_stream(wfv_ctx *ctx, void *hip_stream);
 
+/* ---------------------------------------------------------------------------------------
+   Batches: many independent solver instances with uneven grids advanced together, one
+   persistent launch per RK stage over all of them, each instance with its own time step
+   and step count on the device.  No reference counterpart (the reference runs one case
+   per MPI job); every instance behaves exactly as a single-domain context with the same
+   options would.  Options shared by the batch come from wfv_options (elem_num_x/y are
+   ignored); grid size, boundary and physical case are per instance. */
+typedef struct wfv_instance {
+    int nx, ny;       /* interior cells */
+    int boundary;     /* 0 periodic, 1 reflective walls */
+    int pad;
+    wfv_case cs;
+} wfv_instance;
+
+typedef struct wfv_batch wfv_batch;
+
+int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wfv_batch **out);
+void wfv_batch_destroy(wfv_batch *b);
+const char *wfv_batch_last_error(const wfv_batch *b);
+int wfv_batch_set_state(wfv_batch *b, int i, const double *U);  /* instance i, [4][ny][nx] */
+int wfv_batch_get_state(wfv_batch *b, int i, double *U);
+/* Every instance runs the reference time loop from t = 0 (see wfv_advance): to t_end with
+   the last step clipped, or fixed_steps unclipped CFL steps.  nsteps and t_out (optional)
+   receive each instance's step count and final time; flags (optional) its error bits. */
+int wfv_batch_advance(wfv_batch *b, double t_end, int fixed_steps, int *nsteps, double *t_out,
+                      unsigned *flags);
+int wfv_batch_set_stream(wfv_batch *b, void *hip_stream);
+/* rows per wavefront strip of instance i (the autotuner's choice; 0 keeps the default) */
+int wfv_batch_set_rows_per_strip(wfv_batch *b, int i, int rows);
+int wfv_batch_cells(const wfv_batch *b, long long *cells);  /* total interior cells */
+
 /* Grid of the context: interior cells and the per-column / per-row widths. */
 int wfv_grid(const wfv_ctx *ctx, int *nx, int *ny, double *dxC, double *dxW, double *dyC);
 

@@ -29,22 +29,29 @@ UNITS = [
 ]
 
 
-def build(verbose=False):
+def build(verbose=False, out=LIB, extra_flags=()):
+    """Compile the units in parallel and link `out`; extra_flags (e.g. -D tuning macros)
+    are appended to every unit."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src, extra in UNITS:  # compiled one after another (each unit takes ~10-30 s)
+    objs, procs = [], []
+    for src, extra in UNITS:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + COMMON + extra + list(extra_flags) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"compilation failed: {failed}")
+    tmp = out + ".tmp"
     subprocess.check_call([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    flags = [a for a in sys.argv[1:] if a.startswith("-D")]
+    outs = [a[len("--out="):] for a in sys.argv[1:] if a.startswith("--out=")]
+    print(build(verbose="-v" in sys.argv, out=outs[0] if outs else LIB, extra_flags=flags))

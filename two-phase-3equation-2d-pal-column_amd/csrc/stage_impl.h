@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "wenofv_device.h"
 #include "wenofv_kernels.h"
@@ -108,30 +109,132 @@ WFV_DEV void recon4(const W s[5], W out[4]) {
     out[3] = st.template node<3>();
 }
 
+// Register rotation for loops kept rolled to bound the code size: slot 0 always holds the
+// element of the current iteration, so every register index stays static.
+template <class T>
+WFV_DEV void rotate_left(T (&a)[4]) {
+    const T t = a[0];
+    a[0] = a[1]; a[1] = a[2]; a[2] = a[3]; a[3] = t;
+}
+template <class T>
+WFV_DEV void rotate_push(T (&a)[4], T x) {  // after 4 pushes a[k] holds the k-th value
+    a[0] = a[1]; a[1] = a[2]; a[2] = a[3]; a[3] = x;
+}
+
 // LDS per wavefront.  The y-sweep values are dead once the x-sweep has read them, so the
 // west face values of the same row reuse their space.
-template <class PR>
+template <class W>
 struct WaveLds {
     union {
-        typename PR::W y[16][LANES];     // y-sweep values [g*4+v][lane]
-        typename PR::F west[16][LANES];  // limited west face values [v*4+g][lane]
+        W y[16][LANES];          // y-sweep values [g*4+v][lane] (mixed: deviations)
+        double west[16][LANES];  // limited west face states [v*4+g][lane]
     };
-    double fx[4][LANES];  // integrated flux through the east face [v][lane]
+    double base[4][LANES];  // mixed: reconstructed variables of the centre row [v][lane]
+    double fx[4][LANES];    // integrated flux through the east face [v][lane]
+    double top[16][LANES];  // north face values of the previous row [v*4+g][lane]
 };
 
-template <class PR, int RECON, int SCHEME, int WAVES, int MINW>
-__global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(
-    StageArgs A) {
-    using W = typename PR::W;
-    using F = typename PR::F;
-    __shared__ WaveLds<PR> lds_all[WAVES];
-    const int lane = threadIdx.x & (LANES - 1);
-    const int wave = threadIdx.x / LANES;
-    WaveLds<PR> &L = lds_all[wave];
+// Face values of one cell side, [v][g].  All-FP64: the values themselves.  Mixed: the
+// partial densities in FP64 (they alone enter the equation of state and the limiter's
+// positivity bound, so a limited value of 1e-12 beside a cell average of 0.5 must stay
+// exact) and the momenta as FP32 deviations from the cell's average (the base).
+template <class PR>
+struct FaceSide {
+    double d[4][4];
+    WFV_DEV double get(int v, int g, double) const { return d[v][g]; }
+    WFV_DEV void set(int v, int g, double x, double) { d[v][g] = x; }
+    template <class W>
+    WFV_DEV void set_recon(int v, int g, W w, double) { d[v][g] = double(w); }
+    WFV_DEV void limit(int v, int g, double theta, double ub) { d[v][g] = theta * (d[v][g] - ub) + ub; }
+    WFV_DEV void rotate() {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rotate_left(d[v]);
+    }
+};
+template <>
+struct FaceSide<PrecMixed> {
+    double r[2][4];
+    float m[2][4];
+    WFV_DEV double get(int v, int g, double base) const {
+        return v < 2 ? r[v & 1][g] : base + double(m[v & 1][g]);
+    }
+    WFV_DEV void set(int v, int g, double x, double base) {
+        if (v < 2) r[v & 1][g] = x;
+        else m[v & 1][g] = float(x - base);
+    }
+    // a reconstructed deviation w from the base
+    template <class W>
+    WFV_DEV void set_recon(int v, int g, W w, double base) {
+        if (v < 2) r[v & 1][g] = base + double(w);
+        else m[v & 1][g] = float(w);
+    }
+    WFV_DEV void limit(int v, int g, double theta, double ub) {  // theta*(value - ub) + ub
+        if (v < 2) r[v & 1][g] = theta * (r[v & 1][g] - ub) + ub;
+        else m[v & 1][g] = float(theta * double(m[v & 1][g]));
+    }
+    WFV_DEV void rotate() {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            rotate_left(r[k]);
+            rotate_left(m[k]);
+        }
+    }
+};
 
-    const int strip = blockIdx.x * WAVES + wave;
-    if (strip >= A.strips_x * A.strips_y) return;  // whole wavefront leaves; no block barrier below
-    if (A.active && *A.active == 0) return;         // device-side time loop already finished
+// Riemann flux through one face, integrated over its 4 Gauss-Lobatto nodes
+// (src/WENOFV.cpp:1707-1746 fluxes, 1764-1799 quadrature).  The node loop is rolled: the
+// node values rotate through registers so that one copy of the solver is emitted.
+template <class PR, int FLUX, int NX, int NY>
+WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceSide<PR> &b, const double bb[4],
+                       double width, double G[4], unsigned &err, bool check) {
+    using F = typename PR::F;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) G[v] = 0.0;
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        double ul[4], ur[4];
+        F fl[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            ul[v] = a.get(v, 0, ba[v]);
+            ur[v] = b.get(v, 0, bb[v]);
+        }
+        double pref;
+        riemann<F, FLUX, NX, NY>(P, ul, ur, fl, pref);
+        const double qw = GLW[g] * width / 2.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if (check && isnan(double(fl[v]))) err |= WFV_FLAG_NAN_FLUX;
+            if constexpr (PR::exact) {
+                G[v] += double(fl[v]) * qw;
+            } else {  // add back the reference pressure of the normal momentum flux
+                const double fv = (v == (NX ? 2 : 3)) ? double(fl[v]) + pref : double(fl[v]);
+                G[v] += fv * qw;
+            }
+        }
+        a.rotate();  // node g+1 moves to slot 0
+        b.rotate();
+    }
+}
+
+// reconstruction arithmetic of a policy and form (see stage_body)
+template <class PR, int RECON>
+using ReconT = typename std::conditional<RECON == 1, double, typename PR::W>::type;
+
+// One wavefront's strip of one stage (the whole algorithm; see the file comment).
+template <class PR, int RECON, int SCHEME, int FLUX>
+WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, RECON>> &L) {
+    // Reconstruction arithmetic.  The characteristic form projects with eigenvector
+    // matrices whose condition number reaches ~1e8 in near-pure phases (z1 -> 0 or 1), which
+    // would amplify FP32 rounding of the characteristic values to O(1); the mixed policy
+    // therefore keeps that form's reconstruction in FP64 (its fluxes stay FP32).
+    using W = ReconT<PR, RECON>;
+    using FS = FaceSide<PR>;
+    constexpr bool EX = PR::exact;
+    const int lane = threadIdx.x & (LANES - 1);
+
+    if (strip >= A.strips_x * A.strips_y) return;
+    if (A.active && *A.active == 0) return;  // device-side time loop already finished
     const int sx = strip % A.strips_x, sy = strip / A.strips_x;
     const int nx = A.nx, ny = A.ny;
     const size_t nc = (size_t)nx * ny;
@@ -146,7 +249,6 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
     const bool recon_lane = lane >= 2 && lane <= 61;
     const bool xface_lane = lane >= 2 && lane <= 60;
     const Phys P = A.P;
-    const int flux = A.flux;
     const bool bp = A.bp != 0;
 
     // RK coefficients; c*dt formed as the reference does (c * deltaT, src/WENOFV.cpp:471)
@@ -154,15 +256,13 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
     const double cdt = A.c * dt;
     const double dxc = owned ? A.dxC[cx] : 1.0;
     const double dxw = owned ? A.dxW[cx] : 1.0;
-    double qwx[4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) qwx[g] = GLW[g] * dxc / 2.0;
     constexpr double omega1 = 0.1;
     constexpr double omegaR = 1.0 - 2 * omega1;
 
-    // register window: conservative averages of rows r-2..r+2 of this lane's column
+    // register window: cell averages of rows r-2..r+2 of this lane's column, and their
+    // reconstructed (primitive) variables for RECON 2, 3
     double win[5][4];
-    W pwin[5][4];  // primitive variables of the same rows (RECON 2, 3)
+    double pwin[5][4];
     auto load_row = [&](int row, double out[4]) {
         bool mv;
         const int cy = map_index(row, ny, A.bc, mv);
@@ -171,110 +271,139 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         load_row(y0 - 3 + k, win[k]);
-        if constexpr (RECON >= 2) to_prim<W, RECON>(P, win[k], pwin[k]);
+        if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[k], pwin[k]);
     }
 
-    F top_prev[4][4];      // [v][g] top face values of row r-1
+    // The north face values of row r-1 wait in LDS (L.top) for the face below row r; their
+    // base (mixed: row r-1 cell average) stays in registers.
+    double top_base[4];
     double rhs_part[4];    // RHS of row r-1 without its north face
     unsigned err = 0;
     double dtloc = 1.0e300;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         rhs_part[v] = 0.0;
-#pragma unroll
-        for (int g = 0; g < 4; ++g) top_prev[v][g] = F(0);
+        top_base[v] = 0.0;
     }
 
     for (int it = 0; it <= nrow + 1; ++it) {
         const int r = y0 - 1 + it;
         if (it > 0) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     win[k][v] = win[k + 1][v];
                     if constexpr (RECON >= 2) pwin[k][v] = pwin[k + 1][v];
                 }
-            }
             load_row(r + 2, win[4]);
-            if constexpr (RECON >= 2) to_prim<W, RECON>(P, win[4], pwin[4]);
+            if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[4], pwin[4]);
         }
+        const double *Uc = win[2];                          // cell (xc, r)
+        const double *Bc = (RECON >= 2) ? pwin[2] : win[2];  // its reconstructed variables
 
         // ---- 1. y-direction reconstruction of this lane's column at row r
         //      (getFlux_* step 3, src/WENOFV.cpp:670-687; characteristic 796-833)
         {
-            W yv[4][4];  // [g][v]
+            auto sv = [&](int t, int v) -> W {  // stencil value (mixed: deviation from row r)
+                const double x = (RECON >= 2) ? pwin[t][v] : win[t][v];
+                if constexpr (EX) return x;
+                else return W(x - Bc[v]);
+            };
             if constexpr (RECON == 1) {
-                W Lm[16], Rm[16], pc[5][4], yc[4][4];
-                eigen_LR<W, false>(P, win[2], Lm, Rm);
+                W Lm[16], Rm[16], pc[5][4], yc[4][4], yv[4];
+                eigen_LR<W, false>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int t = 0; t < 5; ++t) {
-                    W u4[4] = {W(win[t][0]), W(win[t][1]), W(win[t][2]), W(win[t][3])};
+                    W u4[4] = {sv(t, 0), sv(t, 1), sv(t, 2), sv(t, 3)};
                     matvec(Lm, u4, pc[t]);
                 }
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    W s[5] = {pc[0][k], pc[1][k], pc[2][k], pc[3][k], pc[4][k]};
+#pragma unroll 1
+                for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
+                    W s5[5] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]};
                     W o[4];
-                    recon4<W, SCHEME>(s, o);
+                    recon4<W, SCHEME>(s5, o);
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) yc[g][k] = o[g];
+                    for (int g = 0; g < 4; ++g) {
+                        rotate_push(yc[g], o[g]);
+                    }
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) rotate_left(pc[t]);
                 }
 #pragma unroll
-                for (int g = 0; g < 4; ++g) matvec(Rm, yc[g], yv[g]);
+                for (int g = 0; g < 4; ++g) {
+                    matvec(Rm, yc[g], yv);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) L.y[g * 4 + v][lane] = yv[v];
+                }
             } else {
+                W sw[5][4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    W s[5];
+                for (int t = 0; t < 5; ++t)
 #pragma unroll
-                    for (int t = 0; t < 5; ++t) s[t] = (RECON == 0) ? W(win[t][v]) : pwin[t][v];
+                    for (int v = 0; v < 4; ++v) sw[t][v] = sv(t, v);
+#pragma unroll 1
+                for (int v = 0; v < 4; ++v) {  // variable v sits in slot 0
+                    W s5[5] = {sw[0][0], sw[1][0], sw[2][0], sw[3][0], sw[4][0]};
                     W o[4];
-                    recon4<W, SCHEME>(s, o);
+                    recon4<W, SCHEME>(s5, o);
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) yv[g][v] = o[g];
+                    for (int g = 0; g < 4; ++g) L.y[g * 4 + v][lane] = o[g];
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) rotate_left(sw[t]);
                 }
             }
+            if constexpr (!EX) {
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) L.y[g * 4 + v][lane] = yv[g][v];
+                for (int v = 0; v < 4; ++v) L.base[v][lane] = Bc[v];
+            }
         }
         wave_sync();
 
         // ---- 2. x-direction reconstruction, back-conversion and limiting of cell (xc, r)
         //      (src/WENOFV.cpp:690-722, 836-880; limiter 1895-2000)
-        F fw[4][4], fe[4][4], fs[4][4], fn[4][4];  // west/east/south/north [v][g]
+        FS fw, fe, fs, fn;  // west/east/south/north face values
 #pragma unroll
         for (int v = 0; v < 4; ++v)
 #pragma unroll
-            for (int g = 0; g < 4; ++g) fw[v][g] = fe[v][g] = fs[v][g] = fn[v][g] = F(0);
+            for (int g = 0; g < 4; ++g) {
+                fw.set(v, g, 0.0, 0.0);
+                fe.set(v, g, 0.0, 0.0);
+                fs.set(v, g, 0.0, 0.0);
+                fn.set(v, g, 0.0, 0.0);
+            }
         double theta = 1.0;
         if (recon_lane) {
-            // Ugp[gx][gy][v] at the 12 boundary nodes, in W before conversion
-            W ug[4][4][4];
+            // x-stencil value of node row gy, variable v, offset t (mixed: deviation from Bc)
+            auto xv = [&](int gy, int v, int t) -> W {
+                const int l = lane - 2 + t;
+                if constexpr (EX) return L.y[gy * 4 + v][l];
+                else return W(L.base[v][l] - Bc[v]) + L.y[gy * 4 + v][l];
+            };
+            W ug[4][4][4];  // Ugp[gx][gy][v] at the 12 boundary nodes
             if constexpr (RECON == 1) {
                 W Lm[16], Rm[16];
-                eigen_LR<W, true>(P, win[2], Lm, Rm);
+                eigen_LR<W, true>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int gy = 0; gy < 4; ++gy) {
                     W pc[5][4];
 #pragma unroll
                     for (int t = 0; t < 5; ++t) {
-                        W y4[4];
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) y4[v] = L.y[gy * 4 + v][lane - 2 + t];
+                        W y4[4] = {xv(gy, 0, t), xv(gy, 1, t), xv(gy, 2, t), xv(gy, 3, t)};
                         matvec(Lm, y4, pc[t]);
                     }
                     W uc[4][4];  // [gx][k]
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        Weno5<W, SCHEME> st(pc[0][k], pc[1][k], pc[2][k], pc[3][k], pc[4][k]);
-                        uc[0][k] = st.template node<0>();
-                        uc[3][k] = st.template node<3>();
+#pragma unroll 1
+                    for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
+                        Weno5<W, SCHEME> st(pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]);
+                        rotate_push(uc[0], st.template node<0>());
+                        rotate_push(uc[3], st.template node<3>());
                         if (gy == 0 || gy == 3) {
-                            uc[1][k] = st.template node<1>();
-                            uc[2][k] = st.template node<2>();
+                            rotate_push(uc[1], st.template node<1>());
+                            rotate_push(uc[2], st.template node<2>());
                         }
+#pragma unroll
+                        for (int t = 0; t < 5; ++t) rotate_left(pc[t]);
                     }
                     matvec(Rm, uc[0], ug[0][gy]);
                     matvec(Rm, uc[3], ug[3][gy]);
@@ -283,14 +412,13 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
                         matvec(Rm, uc[2], ug[2][gy]);
                     }
                 }
-            } else {
+            } else if constexpr (EX) {
+                // all-FP64: unrolled (a rolled loop would keep every node value live across it)
 #pragma unroll
-                for (int gy = 0; gy < 4; ++gy) {
+                for (int gy = 0; gy < 4; ++gy)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        const int row = gy * 4 + v;
-                        Weno5<W, SCHEME> st(L.y[row][lane - 2], L.y[row][lane - 1], L.y[row][lane],
-                                            L.y[row][lane + 1], L.y[row][lane + 2]);
+                        Weno5<W, SCHEME> st(xv(gy, v, 0), xv(gy, v, 1), xv(gy, v, 2), xv(gy, v, 3), xv(gy, v, 4));
                         ug[0][gy][v] = st.template node<0>();
                         ug[3][gy][v] = st.template node<3>();
                         if (gy == 0 || gy == 3) {
@@ -298,34 +426,56 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
                             ug[2][gy][v] = st.template node<2>();
                         }
                     }
+            } else {
+                double bc[4] = {Bc[0], Bc[1], Bc[2], Bc[3]};
+#pragma unroll 1
+                for (int v = 0; v < 4; ++v) {  // variable v's base sits in bc[0]
+#pragma unroll
+                    for (int gy = 0; gy < 4; ++gy) {
+                        W st5[5];
+#pragma unroll
+                        for (int t = 0; t < 5; ++t) {
+                            const int l = lane - 2 + t;
+                            st5[t] = W(L.base[v][l] - bc[0]) + L.y[gy * 4 + v][l];
+                        }
+                        Weno5<W, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
+                        rotate_push(ug[0][gy], st.template node<0>());
+                        rotate_push(ug[3][gy], st.template node<3>());
+                        if (gy == 0 || gy == 3) {
+                            rotate_push(ug[1][gy], st.template node<1>());
+                            rotate_push(ug[2][gy], st.template node<2>());
+                        }
+                    }
+                    rotate_left(bc);
                 }
             }
-            // faces: left = Ugp[0][g], right = Ugp[3][g], bottom = Ugp[g][0], top = Ugp[g][3]
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                F cw[4], ce[4], cs[4], cn[4];
+            // node values -> face values of the four sides; left = Ugp[0][g], right = Ugp[3][g],
+            // bottom = Ugp[g][0], top = Ugp[g][3]
+            auto to_face = [&](const W w[4], FS &f, int g) {
                 if constexpr (RECON >= 2) {
-                    from_prim<W, F, RECON>(P, ug[0][g], cw);
-                    from_prim<W, F, RECON>(P, ug[3][g], ce);
-                    from_prim<W, F, RECON>(P, ug[g][0], cs);
-                    from_prim<W, F, RECON>(P, ug[g][3], cn);
+                    double pv[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) pv[v] = EX ? double(w[v]) : Bc[v] + double(w[v]);
+                    double cv[4];
+                    from_prim<double, double, RECON>(P, pv, cv);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f.set(v, g, cv[v], Uc[v]);
                 } else {
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        cw[v] = F(ug[0][g][v]); ce[v] = F(ug[3][g][v]);
-                        cs[v] = F(ug[g][0][v]); cn[v] = F(ug[g][3][v]);
-                    }
+                    for (int v = 0; v < 4; ++v) f.set_recon(v, g, w[v], Uc[v]);
                 }
+            };
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    fw[v][g] = cw[v]; fe[v][g] = ce[v]; fs[v][g] = cs[v]; fn[v][g] = cn[v];
-                }
+            for (int g = 0; g < 4; ++g) {
+                to_face(ug[0][g], fw, g);
+                to_face(ug[3][g], fe, g);
+                to_face(ug[g][0], fs, g);
+                to_face(ug[g][3], fn, g);
             }
             if (bp) {
                 // getPositivityPreservingTheta (src/WENOFV.cpp:1936-2000), double in both policies
-                const double *Uc = win[2];
                 Eos e;
-                eos_p<!PR::exact>(P, Uc, e);
+                eos_p<!EX>(P, Uc, e);
                 const double u = Uc[2] / (Uc[0] + Uc[1]);
                 const double v = Uc[3] / (Uc[0] + Uc[1]);
                 const double a1 = fabs(u * 1.0 + v * 0.0) + e.c;
@@ -341,19 +491,19 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         const double cg = omega1 * GLW[g] / 2;
-                        rem -= cg * mu2 * double(fs[k][g]);
-                        rem -= cg * mu2 * double(fn[k][g]);
-                        rem -= cg * mu1 * double(fw[k][g]);
-                        rem -= cg * mu1 * double(fe[k][g]);
+                        rem -= cg * mu2 * fs.get(k, g, uave);
+                        rem -= cg * mu2 * fn.get(k, g, uave);
+                        rem -= cg * mu1 * fw.get(k, g, uave);
+                        rem -= cg * mu1 * fe.get(k, g, uave);
                     }
                     rem /= omegaR;
                     double mloc = rem;
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
-                        mloc = scMin(mloc, double(fs[k][g]));
-                        mloc = scMin(mloc, double(fn[k][g]));
-                        mloc = scMin(mloc, double(fw[k][g]));
-                        mloc = scMin(mloc, double(fe[k][g]));
+                        mloc = scMin(mloc, fs.get(k, g, uave));
+                        mloc = scMin(mloc, fn.get(k, g, uave));
+                        mloc = scMin(mloc, fw.get(k, g, uave));
+                        mloc = scMin(mloc, fe.get(k, g, uave));
                     }
                     th[k] = scMin(fabs((uave - eps) / (uave - mloc)), 1.0);
                 }
@@ -364,10 +514,10 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
                         const double ub = Uc[v];
 #pragma unroll
                         for (int g = 0; g < 4; ++g) {
-                            fe[v][g] = F(theta * (double(fe[v][g]) - ub) + ub);
-                            fw[v][g] = F(theta * (double(fw[v][g]) - ub) + ub);
-                            fn[v][g] = F(theta * (double(fn[v][g]) - ub) + ub);
-                            fs[v][g] = F(theta * (double(fs[v][g]) - ub) + ub);
+                            fe.limit(v, g, theta, ub);
+                            fw.limit(v, g, theta, ub);
+                            fn.limit(v, g, theta, ub);
+                            fs.limit(v, g, theta, ub);
                         }
                     }
                 } else if (owned && r >= y0 && r < y1) {
@@ -380,10 +530,10 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
                 for (int v = 0; v < 4; ++v)
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
-                        fo[0 * 16 + v * 4 + g] = double(fw[v][g]);
-                        fo[1 * 16 + v * 4 + g] = double(fe[v][g]);
-                        fo[2 * 16 + v * 4 + g] = double(fs[v][g]);
-                        fo[3 * 16 + v * 4 + g] = double(fn[v][g]);
+                        fo[0 * 16 + v * 4 + g] = fw.get(v, g, Uc[v]);
+                        fo[1 * 16 + v * 4 + g] = fe.get(v, g, Uc[v]);
+                        fo[2 * 16 + v * 4 + g] = fs.get(v, g, Uc[v]);
+                        fo[3 * 16 + v * 4 + g] = fn.get(v, g, Uc[v]);
                     }
                 if (A.theta_out) A.theta_out[(size_t)r * nx + xc] = theta;
             }
@@ -391,50 +541,39 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
 
         if (it > 0) {
             const bool interior_row = it <= nrow;
-            // ---- 3. face below row r: Riemann problem (top of r-1 | bottom of r) and
-            //      Gauss-Lobatto quadrature (src/WENOFV.cpp:1728-1746, 1779-1799)
+            // ---- 3. face below row r: Riemann problem (top of r-1 | bottom of r)
             double G[4] = {0.0, 0.0, 0.0, 0.0};
             if (owned) {
+                FS top_prev;
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    F ul[4], ur[4], fl[4];
+                for (int v = 0; v < 4; ++v)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) { ul[v] = top_prev[v][g]; ur[v] = fs[v][g]; }
-                    riemann<F, 0, 1>(P, flux, ul, ur, fl);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        if (isnan(double(fl[v]))) err |= WFV_FLAG_NAN_FLUX;
-                        G[v] += double(fl[v]) * qwx[g];
-                    }
-                }
+                    for (int g = 0; g < 4; ++g) top_prev.set(v, g, L.top[v * 4 + g][lane], top_base[v]);
+                face_flux<PR, FLUX, 0, 1>(P, top_prev, top_base, fs, Uc, dxc, G, err, true);
             }
             double rhs_x[4] = {0.0, 0.0, 0.0, 0.0};
             double dyr = 1.0;
             if (interior_row) {
                 // ---- 4. face east of cell (xc, r): right face | neighbour lane's left face
-                //      (src/WENOFV.cpp:1707-1725, 1764-1778)
                 wave_sync();  // every lane has read the y-sweep values the west faces overwrite
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw[v][g];
+                    for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
                 wave_sync();
                 dyr = A.dyC[r];
                 if (xface_lane) {
-                    double Fx[4] = {0.0, 0.0, 0.0, 0.0};
+                    // the neighbour's west face values, as exact FP64 states (base 0)
+                    FS nb;  // the neighbour's west face values; momenta relative to its node 0
+                    double nbase[4];
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const double qw = GLW[g] * dyr / 2.0;
-                        F ul[4], ur[4], fl[4];
+                    for (int v = 0; v < 4; ++v) {
+                        nbase[v] = EX ? 0.0 : L.west[v * 4 + 0][lane + 1];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) { ul[v] = fe[v][g]; ur[v] = L.west[v * 4 + g][lane + 1]; }
-                        riemann<F, 1, 0>(P, flux, ul, ur, fl);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            if (owned && isnan(double(fl[v]))) err |= WFV_FLAG_NAN_FLUX;
-                            Fx[v] += double(fl[v]) * qw;
-                        }
+                        for (int g = 0; g < 4; ++g) nb.set(v, g, L.west[v * 4 + g][lane + 1], nbase[v]);
                     }
+                    double Fx[4];
+                    face_flux<PR, FLUX, 1, 0>(P, fe, Uc, nb, nbase, dyr, Fx, err, owned);
 #pragma unroll
                     for (int v = 0; v < 4; ++v) L.fx[v][lane] = Fx[v];
                 }
@@ -475,7 +614,7 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
                 if (A.dtmin) {
                     // calculateDeltaT on the new state (src/WENOFV.cpp:489-513)
                     Eos e;
-                    eos_p<!PR::exact>(P, Unew, e);
+                    eos_p<!EX>(P, Unew, e);
                     const double u = Unew[2] / (Unew[0] + Unew[1]);
                     const double v = Unew[3] / (Unew[0] + Unew[1]);
                     const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
@@ -491,9 +630,11 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
             }
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
+        for (int v = 0; v < 4; ++v) {
+            top_base[v] = Uc[v];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) top_prev[v][g] = fn[v][g];
+            for (int g = 0; g < 4; ++g) L.top[v * 4 + g][lane] = fn.get(v, g, Uc[v]);
+        }
     }
 
     // wavefront reductions: error bits and the CFL minimum
@@ -504,6 +645,71 @@ __global__ __launch_bounds__(LANES *WAVES) __attribute__((amdgpu_waves_per_eu(MI
     if (lane == 0) {
         if (err && A.flags) atomicOr(A.flags, err);
         if (A.dtmin && dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
+    }
+    wave_sync();  // the next strip of a persistent wavefront reuses the LDS
+}
+
+// single domain: one wavefront per strip
+template <class PR, int RECON, int SCHEME, int FLUX, int MINW>
+__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {
+    __shared__ WaveLds<ReconT<PR, RECON>> L;
+    stage_body<PR, RECON, SCHEME, FLUX>(A, blockIdx.x, L);
+}
+
+// Batch of independent instances with uneven grids in one persistent launch: the
+// wavefronts stride over the flattened list of (instance, strip) items; each instance
+// keeps its own grid, parameters, RK registers and device-side time step.
+// Batch of independent instances with uneven grids in one launch: the wavefronts take the
+// items of the flattened (instance, strip) list; each instance keeps its own grid,
+// parameters, RK registers and device-side time step.  PERSIST: a grid smaller than the
+// item list strides over it (a persistent launch); otherwise one wavefront per item.
+template <class PR, int RECON, int SCHEME, int FLUX>
+WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON>> &L) {
+    const int i = B.item_inst[item];
+    const BatchInst &I = B.inst[i];
+    StepCtl *ctl = I.ctl;
+    StageArgs A;
+    A.U = I.reg[B.in];
+    A.Un = B.base < 0 ? nullptr : I.reg[B.base];
+    A.Uo = I.reg[B.out];
+    A.dxC = I.dxC;
+    A.dxW = I.dxW;
+    A.dyC = I.dyC;
+    A.dt_ptr = &ctl->dt;
+    A.active = &ctl->active;
+    A.dt = 0.0;
+    A.a = B.a;
+    A.b = B.b;
+    A.c = B.c;
+    A.P = I.P;
+    A.dX = I.dX;
+    A.dY = I.dY;
+    A.cfl = I.cfl;
+    A.gravity = I.gravity;
+    A.nx = I.nx;
+    A.ny = I.ny;
+    A.bc = I.bc;
+    A.flux = FLUX;
+    A.bp = B.bp;
+    A.ty = I.ty;
+    A.strips_x = I.strips_x;
+    A.strips_y = I.strips_y;
+    A.flags = &ctl->flags;
+    A.dtmin = B.want_dtmin ? &ctl->dtmin_bits : nullptr;
+    A.faces_out = nullptr;
+    A.theta_out = nullptr;
+    stage_body<PR, RECON, SCHEME, FLUX>(A, item - I.item_begin, L);
+}
+
+template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST>
+__global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_stage_kernel(
+    BatchStageArgs B) {
+    __shared__ WaveLds<ReconT<PR, RECON>> L;
+    if constexpr (PERSIST) {
+        for (int item = blockIdx.x; item < B.total_items; item += gridDim.x)
+            batch_item<PR, RECON, SCHEME, FLUX>(B, item, L);
+    } else {
+        if (blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX>(B, blockIdx.x, L);
     }
 }
 

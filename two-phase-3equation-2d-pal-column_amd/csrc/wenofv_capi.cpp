@@ -7,6 +7,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -407,6 +409,317 @@ int wfv_grid(const wfv_ctx *c, int *nx, int *ny, double *dxC, double *dxW, doubl
     if (dxC) std::memcpy(dxC, c->dxC.data(), c->nx * sizeof(double));
     if (dxW) std::memcpy(dxW, c->dxW.data(), c->nx * sizeof(double));
     if (dyC) std::memcpy(dyC, c->dyC.data(), c->ny * sizeof(double));
+    return WFV_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================= batches
+
+struct wfv_batch {
+    wfv_options opt{};
+    int n = 0;
+    std::vector<wfv_instance> inst;
+    std::vector<BatchInst> hinst;   // host copy of the instance table
+    std::vector<size_t> cell_off;   // first cell of each instance in the registers
+    std::vector<int> ty;            // rows per strip per instance
+    size_t total_cells = 0;
+    int total_items = 0;
+    int max_cells = 0;
+    double *reg[3] = {nullptr, nullptr, nullptr};
+    double *geo = nullptr;          // all instances' dxC, dxW, dyC
+    StepCtl *d_ctl = nullptr;
+    BatchInst *d_inst = nullptr;
+    int *d_item_inst = nullptr;
+    hipStream_t stream = nullptr, own_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int grid = 0;                   // persistent wavefronts per stage launch
+    std::string err;
+};
+
+namespace {
+
+int bfail(wfv_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    return code;
+}
+#define BHIPCHK(b, call)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return bfail(b, WFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// (re)build the instance table and the item -> instance map from b->ty
+int batch_layout(wfv_batch *b) {
+    std::vector<int> item_inst;
+    for (int i = 0; i < b->n; ++i) {
+        BatchInst &I = b->hinst[i];
+        I.ty = b->ty[i];
+        I.strips_x = (I.nx + strip_width() - 1) / strip_width();
+        I.strips_y = (I.ny + I.ty - 1) / I.ty;
+        I.item_begin = (int)item_inst.size();
+        for (int k = 0; k < I.strips_x * I.strips_y; ++k) item_inst.push_back(i);
+    }
+    b->total_items = (int)item_inst.size();
+    (void)hipFree(b->d_item_inst);
+    b->d_item_inst = nullptr;
+    BHIPCHK(b, hipMalloc(&b->d_item_inst, item_inst.size() * sizeof(int)));
+    BHIPCHK(b, hipMemcpy(b->d_item_inst, item_inst.data(), item_inst.size() * sizeof(int), hipMemcpyHostToDevice));
+    BHIPCHK(b, hipMemcpy(b->d_inst, b->hinst.data(), b->n * sizeof(BatchInst), hipMemcpyHostToDevice));
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // one wavefront per item by default; WENOFV_BATCH_PERSIST=1 selects the persistent
+    // form (8 wavefronts per CU striding over the items)
+    const char *pe = getenv("WENOFV_BATCH_PERSIST");
+    b->grid = (pe && pe[0] == '1') ? std::min(b->total_items, cus * 8) : b->total_items;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return WFV_OK;
+}
+
+int batch_enqueue_step(wfv_batch *b) {
+    StagePlan pl[3];
+    const int ns = rk_plan(b->opt.rk_method, pl);
+    BHIPCHK(b, launch_batch_step_control(b->d_ctl, b->n, b->stream));
+    for (int s = 0; s < ns; ++s) {
+        BatchStageArgs B{};
+        B.inst = b->d_inst;
+        B.item_inst = b->d_item_inst;
+        B.total_items = b->total_items;
+        B.in = pl[s].in;
+        B.base = pl[s].base;
+        B.out = pl[s].out;
+        B.a = pl[s].a;
+        B.b = pl[s].b;
+        B.c = pl[s].c;
+        B.bp = b->opt.pp_limiter;
+        B.want_dtmin = (s == ns - 1 && b->opt.rk_method != 0) ? 1 : 0;
+        BHIPCHK(b, launch_batch_stage(B, b->opt.precision, b->opt.reconstruction, b->opt.scheme,
+                                      b->opt.riemann_flux, b->grid, b->stream));
+    }
+    if (b->opt.rk_method == 0) {
+        for (int i = 0; i < b->n; ++i)
+            BHIPCHK(b, hipMemcpyAsync(b->hinst[i].reg[0], b->hinst[i].reg[1],
+                                      4 * (size_t)b->inst[i].nx * b->inst[i].ny * sizeof(double),
+                                      hipMemcpyDeviceToDevice, b->stream));
+        BHIPCHK(b, launch_batch_dt(b->d_inst, b->n, b->max_cells, b->stream));
+    }
+    return WFV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wfv_batch **out) {
+    if (!opt || !inst || n < 1 || !out) return WFV_ERR_ARG;
+    *out = nullptr;
+    std::string why;
+    if (opt->rk_method < 0 || opt->rk_method > 2) why = "RKMETH must be 0, 1 or 2";
+    else if (opt->scheme < 0 || opt->scheme > 1) why = "SCHEME must be 0 or 1";
+    else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
+    else if (opt->reconstruction < 0 || opt->reconstruction > 3) why = "RECONSTRUCTIONTYPE must be 0..3";
+    else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 or 1";
+    else if (!(opt->cfl > 0)) why = "CFL must be positive";
+    for (int i = 0; i < n && why.empty(); ++i) {
+        const wfv_instance &I = inst[i];
+        if (I.nx < 3 || I.ny < 3) why = "instance grids need at least 3 cells per direction";
+        else if (I.boundary < 0 || I.boundary > 1) why = "instance boundary must be 0 or 1";
+        else if (!(I.cs.xR > I.cs.xL) || !(I.cs.yR > I.cs.yL)) why = "empty instance domain";
+        else if (opt->geometry_ranks > 1 && I.nx % opt->geometry_ranks) why = "nx not divisible by geometry_ranks";
+    }
+    if (!why.empty()) {
+        g_create_error = why;
+        return WFV_ERR_CONFIG;
+    }
+    wfv_batch *b = new wfv_batch();
+    b->opt = *opt;
+    b->n = n;
+    b->inst.assign(inst, inst + n);
+    b->hinst.resize(n);
+    b->cell_off.resize(n);
+    b->ty.resize(n);
+    size_t geo_total = 0;
+    for (int i = 0; i < n; ++i) {
+        b->cell_off[i] = b->total_cells;
+        b->total_cells += (size_t)inst[i].nx * inst[i].ny;
+        geo_total += 2 * (size_t)inst[i].nx + inst[i].ny;
+        b->max_cells = std::max(b->max_cells, inst[i].nx * inst[i].ny);
+        b->ty[i] = opt->rows_per_strip > 0 ? opt->rows_per_strip : default_rows_per_strip(inst[i].nx, inst[i].ny);
+    }
+    auto bad = [&](hipError_t e) {
+        if (e == hipSuccess) return false;
+        g_create_error = hipGetErrorString(e);
+        wfv_batch_destroy(b);
+        return true;
+    };
+    if (bad(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking))) return WFV_ERR_HIP;
+    b->stream = b->own_stream;
+    for (int k = 0; k < 3; ++k)
+        if (bad(hipMalloc(&b->reg[k], 4 * b->total_cells * sizeof(double)))) return WFV_ERR_HIP;
+    if (bad(hipMalloc(&b->geo, geo_total * sizeof(double))) || bad(hipMalloc(&b->d_ctl, n * sizeof(StepCtl))) ||
+        bad(hipMalloc(&b->d_inst, n * sizeof(BatchInst))))
+        return WFV_ERR_HIP;
+    if (bad(hipMemset(b->reg[0], 0, 4 * b->total_cells * sizeof(double)))) return WFV_ERR_HIP;
+    std::vector<double> geo(geo_total);
+    size_t go = 0;
+    for (int i = 0; i < n; ++i) {
+        const wfv_instance &in = inst[i];
+        BatchInst &I = b->hinst[i];
+        std::memset(&I, 0, sizeof I);
+        for (int k = 0; k < 3; ++k) I.reg[k] = b->reg[k] + 4 * b->cell_off[i];
+        double *dxC = geo.data() + go, *dxW = dxC + in.nx, *dyC = dxW + in.nx;
+        grid_widths(in.nx, in.ny, in.cs.xL, in.cs.xR, in.cs.yL, in.cs.yR,
+                    opt->geometry_ranks < 1 ? 1 : opt->geometry_ranks, dxC, dxW, dyC, nullptr, nullptr);
+        I.dxC = b->geo + go;
+        I.dxW = I.dxC + in.nx;
+        I.dyC = I.dxW + in.nx;
+        go += 2 * (size_t)in.nx + in.ny;
+        I.ctl = b->d_ctl + i;
+        I.P.rho10 = in.cs.rho10;
+        I.P.rho20 = in.cs.rho20;
+        I.P.c1_2 = in.cs.sound1 * in.cs.sound1;
+        I.P.c2_2 = in.cs.sound2 * in.cs.sound2;
+        I.P.preb = in.cs.preb;
+        I.P.q = in.cs.rho20 * I.P.c2_2 - in.cs.rho10 * I.P.c1_2;
+        I.dX = (in.cs.xR - in.cs.xL) / in.nx;
+        I.dY = (in.cs.yR - in.cs.yL) / in.ny;
+        I.cfl = opt->cfl;
+        I.gravity = in.cs.gravity;
+        I.nx = in.nx;
+        I.ny = in.ny;
+        I.bc = in.boundary;
+    }
+    if (bad(hipMemcpy(b->geo, geo.data(), geo_total * sizeof(double), hipMemcpyHostToDevice))) return WFV_ERR_HIP;
+    std::vector<StepCtl> ctl(n);
+    for (auto &c : ctl) {
+        std::memset(&c, 0, sizeof c);
+        c.dtmin_bits = ~0ull;
+    }
+    if (bad(hipMemcpy(b->d_ctl, ctl.data(), n * sizeof(StepCtl), hipMemcpyHostToDevice))) return WFV_ERR_HIP;
+    if (batch_layout(b) != WFV_OK) {
+        g_create_error = b->err;
+        wfv_batch_destroy(b);
+        return WFV_ERR_HIP;
+    }
+    *out = b;
+    return WFV_OK;
+}
+
+void wfv_batch_destroy(wfv_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto &p : b->reg) (void)hipFree(p);
+    (void)hipFree(b->geo);
+    (void)hipFree(b->d_ctl);
+    (void)hipFree(b->d_inst);
+    (void)hipFree(b->d_item_inst);
+    if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
+    delete b;
+}
+
+const char *wfv_batch_last_error(const wfv_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int wfv_batch_set_state(wfv_batch *b, int i, const double *U) {
+    if (!b || !U || i < 0 || i >= b->n) return WFV_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    BHIPCHK(b, hipMemcpy(b->hinst[i].reg[0], U, 4 * (size_t)b->inst[i].nx * b->inst[i].ny * sizeof(double),
+                         hipMemcpyHostToDevice));
+    return WFV_OK;
+}
+
+int wfv_batch_get_state(wfv_batch *b, int i, double *U) {
+    if (!b || !U || i < 0 || i >= b->n) return WFV_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    BHIPCHK(b, hipMemcpy(U, b->hinst[i].reg[0], 4 * (size_t)b->inst[i].nx * b->inst[i].ny * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return WFV_OK;
+}
+
+int wfv_batch_set_stream(wfv_batch *b, void *hip_stream) {
+    if (!b) return WFV_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->stream = hip_stream ? (hipStream_t)hip_stream : b->own_stream;
+    return WFV_OK;
+}
+
+int wfv_batch_set_rows_per_strip(wfv_batch *b, int i, int rows) {
+    if (!b || i < 0 || i >= b->n || rows < 0) return WFV_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    b->ty[i] = rows > 0 ? rows : default_rows_per_strip(b->inst[i].nx, b->inst[i].ny);
+    return batch_layout(b);
+}
+
+int wfv_batch_cells(const wfv_batch *b, long long *cells) {
+    if (!b || !cells) return WFV_ERR_ARG;
+    *cells = (long long)b->total_cells;
+    return WFV_OK;
+}
+
+int wfv_batch_advance(wfv_batch *b, double t_end, int fixed_steps, int *nsteps, double *t_out, unsigned *flags) {
+    if (!b) return WFV_ERR_ARG;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    std::vector<StepCtl> ctl(b->n);
+    BHIPCHK(b, hipMemcpy(ctl.data(), b->d_ctl, b->n * sizeof(StepCtl), hipMemcpyDeviceToHost));
+    for (auto &c : ctl) {
+        c.dtmin_bits = ~0ull;
+        c.now = 0.0;
+        c.t_end = t_end;
+        c.count = 0;
+        c.fixed_steps = fixed_steps;
+        c.active = 1;
+        c.dts = nullptr;
+        c.max_record = 0;
+    }
+    BHIPCHK(b, hipMemcpy(b->d_ctl, ctl.data(), b->n * sizeof(StepCtl), hipMemcpyHostToDevice));
+    BHIPCHK(b, launch_batch_dt(b->d_inst, b->n, b->max_cells, b->stream));
+    constexpr int GRAPH_STEPS = 16;
+    if (!b->graph) {
+        hipGraph_t g;
+        BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = WFV_OK;
+        for (int k = 0; k < GRAPH_STEPS && rc == WFV_OK; ++k) rc = batch_enqueue_step(b);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != WFV_OK) return rc;
+        if (e != hipSuccess) return bfail(b, WFV_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return bfail(b, WFV_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+    }
+    if (fixed_steps > 0) {
+        const int full = fixed_steps / GRAPH_STEPS, rest = fixed_steps % GRAPH_STEPS;
+        for (int k = 0; k < full; ++k) BHIPCHK(b, hipGraphLaunch(b->graph, b->stream));
+        for (int k = 0; k < rest; ++k) {
+            int rc = batch_enqueue_step(b);
+            if (rc) return rc;
+        }
+        BHIPCHK(b, hipStreamSynchronize(b->stream));
+        BHIPCHK(b, hipMemcpy(ctl.data(), b->d_ctl, b->n * sizeof(StepCtl), hipMemcpyDeviceToHost));
+    } else {
+        for (;;) {
+            for (int k = 0; k < 4; ++k) BHIPCHK(b, hipGraphLaunch(b->graph, b->stream));
+            BHIPCHK(b, hipStreamSynchronize(b->stream));
+            BHIPCHK(b, hipMemcpy(ctl.data(), b->d_ctl, b->n * sizeof(StepCtl), hipMemcpyDeviceToHost));
+            bool any = false;
+            for (auto &c : ctl) any |= c.active != 0;
+            if (!any) break;
+        }
+    }
+    bool nan_dt = false;
+    for (int i = 0; i < b->n; ++i) {
+        if (nsteps) nsteps[i] = ctl[i].count;
+        if (t_out) t_out[i] = ctl[i].now;
+        if (flags) flags[i] = ctl[i].flags;
+        nan_dt |= (ctl[i].flags & WFV_FLAG_NAN_DT) != 0;
+        ctl[i].flags = 0;
+    }
+    BHIPCHK(b, hipMemcpy(b->d_ctl, ctl.data(), b->n * sizeof(StepCtl), hipMemcpyHostToDevice));
+    if (nan_dt) return bfail(b, WFV_ERR_NONFINITE_DT, "Error: exist nan DeltaT");
     return WFV_OK;
 }
 

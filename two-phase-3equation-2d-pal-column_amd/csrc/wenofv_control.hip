@@ -73,6 +73,93 @@ __global__ void step_control_kernel(StepCtl *S) {
     S->dtmin_bits = ~0ull;
 }
 
+// step control of every instance of a batch (one thread per instance)
+__global__ void batch_step_control_kernel(StepCtl *ctl, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StepCtl *S = ctl + i;
+    if (S->active == 0) return;
+    bool go;
+    if (S->fixed_steps > 0) go = S->count < S->fixed_steps;
+    else go = fabs(S->now - S->t_end) > 1e-10;
+    if (!go || (S->flags & WFV_FLAG_NAN_DT)) {
+        S->active = 0;
+        return;
+    }
+    double timestep = 1.0;
+    const double m = __longlong_as_double((long long)S->dtmin_bits);
+    if (S->dtmin_bits != ~0ull && !(m > timestep)) timestep = m;
+    if (S->fixed_steps <= 0) timestep = stdMin(timestep, S->t_end - S->now);
+    S->dt = timestep;
+    if (S->count < S->max_record && S->dts) S->dts[S->count] = timestep;
+    S->now += timestep;
+    S->count += 1;
+    S->dtmin_bits = ~0ull;
+}
+
+// CFL minimum of every instance's state (blockIdx.y = instance)
+__global__ __launch_bounds__(DT_THREADS) void batch_dt_kernel(const BatchInst *inst) {
+    __shared__ double smin[DT_THREADS / 64];
+    __shared__ unsigned serr[DT_THREADS / 64];
+    const BatchInst &I = inst[blockIdx.y];
+    const size_t nc = (size_t)I.nx * I.ny;
+    const double *U = I.reg[0];
+    double dtloc = 1.0e300;
+    unsigned err = 0;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += (size_t)gridDim.x * blockDim.x) {
+        const double u4[4] = {U[c], U[nc + c], U[2 * nc + c], U[3 * nc + c]};
+        Eos e;
+        eos(I.P, u4, e);
+        const double u = u4[2] / (u4[0] + u4[1]);
+        const double v = u4[3] / (u4[0] + u4[1]);
+        const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
+        const double ey = fabs(u * 0.0 + v * 1.0) + e.c;
+        const double tmp = I.cfl / (ex / I.dX + ey / I.dY);
+        if (isnan(tmp)) err |= WFV_FLAG_NAN_DT;
+        else dtloc = fmin(dtloc, tmp);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        err |= __shfl_xor(err, off, 64);
+        dtloc = fmin(dtloc, __shfl_xor(dtloc, off, 64));
+    }
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        smin[w] = dtloc;
+        serr[w] = err;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < DT_THREADS / 64; ++k) {
+            dtloc = fmin(dtloc, smin[k]);
+            err |= serr[k];
+        }
+        if (err) atomicOr(&I.ctl->flags, err);
+        if (dtloc < 1.0e300) atomicMin(&I.ctl->dtmin_bits, (unsigned long long)__double_as_longlong(dtloc));
+    }
+}
+
+hipError_t launch_batch_step_control(StepCtl *ctl, int n, hipStream_t s) {
+    hipLaunchKernelGGL(batch_step_control_kernel, dim3((n + 63) / 64), dim3(64), 0, s, ctl, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_dt(const BatchInst *inst, int n, int max_cells, hipStream_t s) {
+    int bx = (max_cells + DT_THREADS - 1) / DT_THREADS;
+    if (bx > 64) bx = 64;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(batch_dt_kernel, dim3(bx, n), dim3(DT_THREADS), 0, s, inst);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_fp64(const BatchStageArgs &B, int recon, int scheme, int flux, int grid, hipStream_t s);
+hipError_t launch_batch_mixed(const BatchStageArgs &B, int recon, int scheme, int flux, int grid, hipStream_t s);
+
+hipError_t launch_batch_stage(const BatchStageArgs &B, int precision, int recon, int scheme, int flux, int grid,
+                              hipStream_t s) {
+    if (precision == 1) return launch_batch_mixed(B, recon, scheme, flux, grid, s);
+    return launch_batch_fp64(B, recon, scheme, flux, grid, s);
+}
+
 hipError_t launch_stage_fp64(const StageArgs &A, int recon, int scheme, hipStream_t s);
 hipError_t launch_stage_mixed(const StageArgs &A, int recon, int scheme, hipStream_t s);
 

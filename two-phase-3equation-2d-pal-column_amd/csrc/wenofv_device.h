@@ -128,14 +128,22 @@ WFV_DEV void phy_flux(const T U[4], T rho, T u, T v, T p, T F[4]) {
     }
 }
 
+// In the float policy the fluxes are returned minus pref * n with pref the left state's
+// pressure: the O(1) pressure term of the momentum flux would otherwise dominate its FP32
+// rounding, while its constant part cancels exactly in the divergence once added back in
+// double (the caller adds pref * n).  State jumps are taken in double before rounding.
+// The double policy keeps the reference's formulas unchanged (pref = 0).
+
 template <class T, int NX, int NY>
-WFV_DEV void riemann_llf(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
+WFV_DEV void riemann_llf(const Phys &P, const double dl[4], const double dr[4], T F[4], double &pref) {
     // getLLFRiemannFlux (include/Equations.hpp:363-378)
-    double dl[4] = {(double)UL[0], (double)UL[1], (double)UL[2], (double)UL[3]};
-    double dr[4] = {(double)UR[0], (double)UR[1], (double)UR[2], (double)UR[3]};
+    constexpr bool EXACT = sizeof(T) == 8;
+    const T UL[4] = {T(dl[0]), T(dl[1]), T(dl[2]), T(dl[3])};
+    const T UR[4] = {T(dr[0]), T(dr[1]), T(dr[2]), T(dr[3])};
     Eos el, er;
-    eos_p<sizeof(T) == 4>(P, dl, el);
-    eos_p<sizeof(T) == 4>(P, dr, er);
+    eos_p<!EXACT>(P, dl, el);
+    eos_p<!EXACT>(P, dr, er);
+    pref = EXACT ? 0.0 : el.p;
     const T nx = T(NX), ny = T(NY);
     const T rl = T(el.rho), rr = T(er.rho);
     const T ul = dv(UL[2], rl), vl = dv(UL[3], rl);
@@ -144,25 +152,30 @@ WFV_DEV void riemann_llf(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
     const T wsR = fabs(ur * nx + vr * ny) + T(er.c);
     const T ws = stdMax(wsL, wsR);
     T FL[4], FR[4];
-    phy_flux<T, NX, NY>(UL, rl, ul, vl, T(el.p), FL);
-    phy_flux<T, NX, NY>(UR, rr, ur, vr, T(er.p), FR);
+    phy_flux<T, NX, NY>(UL, rl, ul, vl, T(el.p - pref), FL);
+    phy_flux<T, NX, NY>(UR, rr, ur, vr, T(er.p - pref), FR);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) F[r] = T(0.5) * (FL[r] + FR[r] - ws * (UR[r] - UL[r]));
+    for (int r = 0; r < 4; ++r) {
+        if constexpr (EXACT) F[r] = T(0.5) * (FL[r] + FR[r] - ws * (UR[r] - UL[r]));
+        else F[r] = T(0.5) * (FL[r] + FR[r] - ws * T(dr[r] - dl[r]));
+    }
 }
 
 template <class T, int NX, int NY>
-WFV_DEV void riemann_hllc(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
+WFV_DEV void riemann_hllc(const Phys &P, const double dl[4], const double dr[4], T F[4], double &pref) {
     // getHLLCRiemannFlux (include/Equations.hpp:381-483)
-    double dl[4] = {(double)UL[0], (double)UL[1], (double)UL[2], (double)UL[3]};
-    double dr[4] = {(double)UR[0], (double)UR[1], (double)UR[2], (double)UR[3]};
+    constexpr bool EXACT = sizeof(T) == 8;
+    const T UL[4] = {T(dl[0]), T(dl[1]), T(dl[2]), T(dl[3])};
+    const T UR[4] = {T(dr[0]), T(dr[1]), T(dr[2]), T(dr[3])};
     Eos el, er;
-    eos_p<sizeof(T) == 4>(P, dl, el);
-    eos_p<sizeof(T) == 4>(P, dr, er);
+    eos_p<!EXACT>(P, dl, el);
+    eos_p<!EXACT>(P, dr, er);
+    pref = EXACT ? 0.0 : el.p;
     const T nx = T(NX), ny = T(NY);
     const T rho_L = T(el.rho), rho_R = T(er.rho);
     const T u_L = dv(UL[2], rho_L), v_L = dv(UL[3], rho_L);
     const T u_R = dv(UR[2], rho_R), v_R = dv(UR[3], rho_R);
-    const T pre_L = T(el.p), pre_R = T(er.p), c_L = T(el.c), c_R = T(er.c);
+    const T pre_L = T(el.p - pref), pre_R = T(er.p - pref), c_L = T(el.c), c_R = T(er.c);
     const T vn_L = u_L * nx + v_L * ny;
     const T vn_R = u_R * nx + v_R * ny;
     T FL[4], FR[4];
@@ -177,6 +190,7 @@ WFV_DEV void riemann_hllc(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
     const T c_roe = sqrt(c2_roe);
     const T S_L = stdMin(vn_L - c_L, vn_roe - c_roe);
     const T S_R = stdMax(vn_R + c_R, vn_roe + c_roe);
+    // with pref = p_L the shifted pressures already form the jump pre_R - pre_L
     const T S_star = dv((S_L - vn_L) * rho_L * vn_L - (S_R - vn_R) * rho_R * vn_R + pre_R - pre_L,
                         (S_L - vn_L) * rho_L - (S_R - vn_R) * rho_R);
     if (S_star >= T(0)) {
@@ -202,11 +216,11 @@ WFV_DEV void riemann_hllc(const Phys &P, const T UL[4], const T UR[4], T F[4]) {
     }
 }
 
-template <class T, int NX, int NY>
-WFV_DEV void riemann(const Phys &P, int type, const T UL[4], const T UR[4], T F[4]) {
-    // getRiemannFlux dispatch (include/Equations.hpp:329-360)
-    if (type == 1) riemann_hllc<T, NX, NY>(P, UL, UR, F);
-    else riemann_llf<T, NX, NY>(P, UL, UR, F);
+// getRiemannFlux dispatch (include/Equations.hpp:329-360), resolved at compile time
+template <class T, int FLUX, int NX, int NY>
+WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], T F[4], double &pref) {
+    if constexpr (FLUX == 1) riemann_hllc<T, NX, NY>(P, UL, UR, F, pref);
+    else riemann_llf<T, NX, NY>(P, UL, UR, F, pref);
 }
 
 // ------------------------------------------------------------------ eigenvectors

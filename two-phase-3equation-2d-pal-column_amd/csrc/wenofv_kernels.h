@@ -56,7 +56,34 @@ struct StepCtl {
     int pad;
 };
 
+// one instance of a batch (device memory)
+struct BatchInst {
+    double *reg[3];              // RK registers; reg[0] is the state
+    const double *dxC, *dxW, *dyC;
+    StepCtl *ctl;                // this instance's time loop
+    Phys P;
+    double dX, dY, cfl, gravity;
+    int nx, ny, bc, ty;
+    int strips_x, strips_y;
+    int item_begin;              // first item of this instance in the flattened list
+    int pad;
+};
+
+// one stage over every instance of a batch
+struct BatchStageArgs {
+    const BatchInst *inst;
+    const int *item_inst;  // [total_items] instance of each item
+    int total_items;
+    int in, base, out;     // RK register indices (base < 0: Un = U)
+    double a, b, c;
+    int bp, want_dtmin;
+};
+
 hipError_t launch_stage(const StageArgs &A, int precision, int recon, int scheme, hipStream_t s);
+hipError_t launch_batch_stage(const BatchStageArgs &B, int precision, int recon, int scheme, int flux, int grid,
+                              hipStream_t s);
+hipError_t launch_batch_step_control(StepCtl *ctl, int n, hipStream_t s);
+hipError_t launch_batch_dt(const BatchInst *inst, int n, int max_cells, hipStream_t s);
 hipError_t launch_dt(const DtArgs &A, hipStream_t s);
 hipError_t launch_step_control(StepCtl *S, hipStream_t s);
 int strip_width();

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwenofv.so")
+LIB_PATH = os.environ.get("WENOFV_LIB", os.path.join(HERE, "libwenofv.so"))
 
 OK, ERR_CONFIG, ERR_HIP, ERR_NONFINITE_DT, ERR_ARG = 0, 1, 2, 3, 4
 FLAG_NAN_FLUX, FLAG_BAD_THETA, FLAG_NAN_DT = 1, 2, 4
@@ -168,3 +168,97 @@ class Solver:
 
 def version():
     return lib().wfv_version().decode()
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("boundary", ctypes.c_int), ("pad", ctypes.c_int),
+                ("cs", Case)]
+
+
+def _batch_lib():
+    L = lib()
+    if getattr(L, "_batch_ready", False):
+        return L
+    vp = ctypes.c_void_p
+    L.wfv_batch_create.argtypes = [ctypes.POINTER(Options), ctypes.POINTER(Instance), ctypes.c_int,
+                                   ctypes.POINTER(vp)]
+    L.wfv_batch_destroy.argtypes = [vp]
+    L.wfv_batch_last_error.restype = ctypes.c_char_p
+    L.wfv_batch_last_error.argtypes = [vp]
+    L.wfv_batch_set_state.argtypes = [vp, ctypes.c_int, vp]
+    L.wfv_batch_get_state.argtypes = [vp, ctypes.c_int, vp]
+    L.wfv_batch_advance.argtypes = [vp, ctypes.c_double, ctypes.c_int, vp, vp, vp]
+    L.wfv_batch_set_stream.argtypes = [vp, vp]
+    L.wfv_batch_set_rows_per_strip.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.wfv_batch_cells.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    L._batch_ready = True
+    return L
+
+
+class Batch:
+    """Independent instances with uneven grids advanced together (wfv_batch_* ABI).
+
+    instances: list of dicts with nx, ny, bc and the case (domain, rho10, rho20, sound1,
+    sound2, preb, gravity); options shared by the batch as keyword arguments."""
+
+    def __init__(self, instances, scheme=0, recon=0, flux=0, bp=0, rk=2, cfl=0.4, precision=0, geometry_ranks=1,
+                 rows_per_strip=0):
+        L = _batch_lib()
+        self.n = len(instances)
+        self.shapes = [(d["ny"], d["nx"]) for d in instances]
+        self.opt = Options(0, 0, rk, cfl, scheme, bp, flux, recon, precision, 0, geometry_ranks, rows_per_strip)
+        arr = (Instance * self.n)()
+        for k, d in enumerate(instances):
+            xL, xR, yL, yR = d.get("domain", (0.0, 1.0, 0.0, 1.0))
+            arr[k] = Instance(d["nx"], d["ny"], d.get("bc", 0), 0,
+                              Case(xL, xR, yL, yR, d["rho10"], d["rho20"], d["sound1"], d["sound2"],
+                                   d.get("preb", 0.0), d.get("gravity", 0.0)))
+        h = ctypes.c_void_p()
+        rc = L.wfv_batch_create(ctypes.byref(self.opt), arr, self.n, ctypes.byref(h))
+        if rc != OK:
+            raise WenofvError(f"wfv_batch_create failed ({rc}): {L.wfv_batch_last_error(None).decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _batch_lib().wfv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise WenofvError(f"wenofv batch error {rc}: {_batch_lib().wfv_batch_last_error(self._h).decode()}")
+
+    def set_state(self, i, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (4,) + self.shapes[i]
+        self._check(_batch_lib().wfv_batch_set_state(self._h, i, U.ctypes.data))
+
+    def get_state(self, i):
+        U = np.empty((4,) + self.shapes[i])
+        self._check(_batch_lib().wfv_batch_get_state(self._h, i, U.ctypes.data))
+        return U
+
+    def advance(self, t_end=0.0, fixed_steps=0):
+        n = np.zeros(self.n, dtype=np.int32)
+        t = np.zeros(self.n)
+        f = np.zeros(self.n, dtype=np.uint32)
+        self._check(_batch_lib().wfv_batch_advance(self._h, t_end, fixed_steps, n.ctypes.data, t.ctypes.data,
+                                                   f.ctypes.data))
+        return n, t, f
+
+    def set_stream(self, stream_handle):
+        self._check(_batch_lib().wfv_batch_set_stream(self._h, ctypes.c_void_p(stream_handle or None)))
+
+    def set_rows_per_strip(self, i, rows):
+        self._check(_batch_lib().wfv_batch_set_rows_per_strip(self._h, i, rows))
+
+    def cells(self):
+        c = ctypes.c_longlong()
+        self._check(_batch_lib().wfv_batch_cells(self._h, ctypes.byref(c)))
+        return c.value
