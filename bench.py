@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="batch256: skip the FP64 baseline run")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy a device buffer of this size once (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -127,6 +129,8 @@ def run_batch(a, torch, wenofv, cases):
     # runs: mixed (timed), all-FP64 (timed; the baseline and the reference state), and
     # all-FP64 from initial data perturbed by 1e-13 (the FP64 solution's own sensitivity)
     runs = [("mixed", 1, False)] + ([] if a.no_fp64 else [("fp64", 0, False), ("fp64_perturbed", 0, True)])
+    if a.no_fp64 and a.precision == 0:  # time the all-FP64 batch alone
+        runs = [("mixed", 0, False)]
     for tag, prec, perturb in runs:
         b = wenofv.Batch(insts, rows_per_strip=a.ty, **dict(opts, precision=prec))
         b.set_stream(stream.cuda_stream)
@@ -244,6 +248,14 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("no GPU visible")
     out = run_batch(a, torch, wenofv, cases) if a.workload == "batch256" else run_single(a, torch, wenofv, cases)
+    if a.copy_calibration_mib:
+        n = a.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": n * 8, "write": n * 8}
     print(json.dumps(out), flush=True)
 
 

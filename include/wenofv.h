@@ -138,6 +138,16 @@ int wfv_batch_cells(const wfv_batch *b, long long *cells);  /* total interior ce
 /* Grid of the context: interior cells and the per-column / per-row widths. */
 int wfv_grid(const wfv_ctx *ctx, int *nx, int *ny, double *dxC, double *dxW, double *dyC);
 
+/* TwoPhaseEquation::setEquationParameters (reference include/Equations.hpp:486-977) of a
+   reference test case (TESTCASE 0..8): its domain and phase parameters, boundary (0
+   periodic, 1 walls for the reference's non-periodic kinds) and final output time.  Host
+   only, no device work. */
+int wfv_reference_case(int testcase, wfv_case *cs, int *boundary, double *t_end);
+/* CWENOFV::initializeAve (reference src/WENOFV.cpp:232-279) of a reference test case: 4x4
+   Gauss-Legendre cell averages of TwoPhaseEquation::getU0 on the grid of geometry_ranks
+   x-slabs; U: [4][ny][nx].  Host only, no device work. */
+int wfv_reference_initial_state(int testcase, int nx, int ny, int geometry_ranks, double *U);
+
 /* Library build identification (kernel variants compiled for gfx950). */
 const char *wfv_version(void);
 

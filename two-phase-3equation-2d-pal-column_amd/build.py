@@ -26,7 +26,9 @@ UNITS = [
     ("wenofv_kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("wenofv_control.hip", ["-ffp-contract=off"]),
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),
+    ("WENOFV.cpp", ["-ffp-contract=off"]),
 ]
+CLI = os.path.join(HERE, "wenofv_run")
 
 
 def build(verbose=False, out=LIB, extra_flags=()):
@@ -48,6 +50,11 @@ def build(verbose=False, out=LIB, extra_flags=()):
     tmp = out + ".tmp"
     subprocess.check_call([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, out)
+    if out == LIB:
+        # command-line driver (config/settings.cfg), linked against the library in place
+        subprocess.check_call([HIPCC] + COMMON + ["-ffp-contract=off", os.path.join(CSRC, "main.cpp"), "-o", CLI + ".tmp",
+                               "-L" + HERE, "-lwenofv", "-Wl,-rpath,$ORIGIN"])
+        os.replace(CLI + ".tmp", CLI)
     return out
 
 

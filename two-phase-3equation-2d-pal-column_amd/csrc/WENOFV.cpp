@@ -1,0 +1,266 @@
+// WENOFV.cpp -- CWENOFV facade (see WENOFV.hpp) over the C ABI.
+#include "WENOFV.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+
+#include "wenofv_geometry.h"
+
+extern "C" int wfv_reference_initial_state(int testcase, int nx, int ny, int geometry_ranks, double *U);
+
+namespace {
+// Gauss-Legendre-4 on [-1,1] (third_party/scTools/GaussQuadratureInRef.hpp:64-75)
+const double GLP[4] = {-0.861136311594052575223946488893, -0.339981043584856264802665759103,
+                       0.339981043584856264802665759103, 0.861136311594052575223946488893};
+const double GLWT[4] = {0.347854845137453857373063949222, 0.652145154862546142626936050778,
+                        0.652145154862546142626936050778, 0.347854845137453857373063949222};
+}  // namespace
+
+void CWENOFV::check(int rc, const char *what) const {
+    if (rc != WFV_OK) throw std::runtime_error(std::string(what) + ": " + wfv_last_error(ctx_));
+}
+
+int CWENOFV::stoi_key(const char *key) {
+    // a missing key makes the reference's std::stoi("") throw (src/WENOFV.cpp:43-63)
+    auto it = option.find(key);
+    if (it == option.end() || it->second.empty()) throw std::invalid_argument(std::string("missing option ") + key);
+    return std::stoi(it->second);
+}
+
+void CWENOFV::initializeSolver() {
+    m_testcase = (TESTCASETYPE)stoi_key("TESTCASE");
+    m_rkMeth = (RKMETHOD)stoi_key("RKMETH");
+    m_riemannFluxType = (RIEMANNFLUXTYPE)stoi_key("RIEMANNFLUX");
+    m_reconstructionType = (RECONSTRUCTIONTYPE)stoi_key("RECONSTRUCTIONTYPE");
+    equation->setEquationParameters(m_testcase);
+    m_timeMoments = equation->outputTimes;
+    m_scheme = (SCHEMETYPE)stoi_key("SCHEME");
+    m_pplimiter = stoi_key("PPlimiter") != 0;
+    auto cfl = option.find("CFL");
+    if (cfl == option.end() || cfl->second.empty()) throw std::invalid_argument("missing option CFL");
+    m_cfl = std::stod(cfl->second);
+    m_worldElemNumX = stoi_key("ELEMNUMX");
+    m_worldElemNumY = stoi_key("ELEMNUMY");
+    m_precision = option.count("PRECISION") ? std::stoi(option["PRECISION"]) : 0;
+    m_deltaX = (equation->xR - equation->xL) / m_worldElemNumX;
+    m_deltaY = (equation->yR - equation->yL) / m_worldElemNumY;
+    if (m_scheme == WENOEXP || m_reconstructionType == PRIMITIVE3)
+        throw std::invalid_argument("SCHEME=2 (WENO-EXP) and RECONSTRUCTIONTYPE=4 are not provided by this build");
+
+    wfv_options o{};
+    o.elem_num_x = m_worldElemNumX;
+    o.elem_num_y = m_worldElemNumY;
+    o.rk_method = m_rkMeth;
+    o.cfl = m_cfl;
+    o.scheme = m_scheme;
+    o.pp_limiter = m_pplimiter ? 1 : 0;
+    o.riemann_flux = m_riemannFluxType;
+    o.reconstruction = m_reconstructionType;
+    o.precision = m_precision;
+    o.boundary = equation->boundaryCondition == PERIOD ? 0 : 1;
+    // the reference splits x over exactly 4 ranks and needs ELEMNUMX % 4 == 0
+    // (src/WENOFV.cpp:24, 71-76); its per-rank cell widths are reproduced when possible
+    o.geometry_ranks = (m_worldElemNumX % 4 == 0) ? 4 : 1;
+    o.rows_per_strip = option.count("ROWSPERSTRIP") ? std::stoi(option["ROWSPERSTRIP"]) : 0;
+    wfv_case c{};
+    c.xL = equation->xL;
+    c.xR = equation->xR;
+    c.yL = equation->yL;
+    c.yR = equation->yR;
+    c.rho10 = equation->g_rho10;
+    c.rho20 = equation->g_rho20;
+    c.sound1 = equation->g_sound1;
+    c.sound2 = equation->g_sound2;
+    c.preb = equation->g_preb;
+    c.gravity = equation->g_gravity;
+    wfv_destroy(ctx_);
+    ctx_ = nullptr;
+    const int rc = wfv_create(&o, &c, &ctx_);
+    if (rc != WFV_OK) throw std::runtime_error(std::string("wfv_create: ") + wfv_last_error(nullptr));
+    m_outputDir = "./output/" + TESTCASETYPE_STRINGS[m_testcase] + "_" + SCHEMETYPE_STRINGS[m_scheme] + "_" +
+                  RECONSTRUCTIONTYPE_STRINGS[m_reconstructionType] + "_" + RIEMANNFLUXTYPE_STRINGS[m_riemannFluxType] +
+                  "_" + std::to_string(m_worldElemNumX) + "x" + std::to_string(m_worldElemNumY) + "/";
+    initializeAve();
+}
+
+void CWENOFV::initializeAve() {
+    m_Uh.assign(4 * (size_t)m_worldElemNumX * m_worldElemNumY, 0.0);
+    check(wfv_reference_initial_state(m_testcase, m_worldElemNumX, m_worldElemNumY,
+                                      (m_worldElemNumX % 4 == 0) ? 4 : 1, m_Uh.data()),
+          "initializeAve");
+    check(wfv_set_state(ctx_, m_Uh.data()), "wfv_set_state");
+}
+
+double CWENOFV::calculateDeltaT(double) {
+    double dt = 0;
+    check(wfv_compute_dt(ctx_, &dt), "calculateDeltaT");
+    return dt;
+}
+
+void CWENOFV::performTimeIntegration(double timestep) { check(wfv_step(ctx_, timestep), "performTimeIntegration"); }
+
+void CWENOFV::rk_checked(RKMETHOD m, double dt) {
+    if (m != m_rkMeth) throw std::invalid_argument("the context was built for RKMETH " + std::to_string(m_rkMeth));
+    performTimeIntegration(dt);
+}
+
+void CWENOFV::assembleRHS() {
+    const size_t nc = (size_t)m_worldElemNumX * m_worldElemNumY;
+    m_rhs.resize(4 * nc);
+    m_faces.resize(64 * nc);
+    m_theta.resize(nc);
+    check(wfv_assemble_rhs(ctx_, m_rhs.data(), m_faces.data(), m_theta.data()), "assembleRHS");
+}
+
+double CWENOFV::getPositivityPreservingTheta(int ei, int ej) const {
+    const int ix = ei - m_ghostCellNum, iy = ej - m_ghostCellNum;
+    if (m_theta.empty() || ix < 0 || iy < 0 || ix >= m_worldElemNumX || iy >= m_worldElemNumY)
+        throw std::out_of_range("getPositivityPreservingTheta: call assembleRHS first / cell outside the domain");
+    return m_theta[(size_t)iy * m_worldElemNumX + ix];
+}
+
+void CWENOFV::getWorldUh() {
+    m_Uh.resize(4 * (size_t)m_worldElemNumX * m_worldElemNumY);
+    check(wfv_get_state(ctx_, m_Uh.data()), "getWorldUh");
+}
+
+std::vector<double> CWENOFV::outputAccuracy(const std::string &prefix, double now) {
+    getWorldUh();
+    const int nx = m_worldElemNumX, ny = m_worldElemNumY;
+    const size_t nc = (size_t)nx * ny;
+    double err1 = 0, err2 = 0, errinf = 0;
+    for (int ei = 0; ei < nx; ++ei)
+        for (int ej = 0; ej < ny; ++ej) {
+            // world grid of rank 0 (src/WENOFV.cpp:127-141): xl = xL - 3 dx + i dx
+            const int wi = ei + m_ghostCellNum, wj = ej + m_ghostCellNum;
+            const double xl = equation->xL - m_ghostCellNum * m_deltaX + wi * m_deltaX;
+            const double xr = equation->xL - m_ghostCellNum * m_deltaX + (wi + 1) * m_deltaX;
+            const double yl = equation->yL - m_ghostCellNum * m_deltaY + wj * m_deltaY;
+            const double yr = equation->yL - m_ghostCellNum * m_deltaY + (wj + 1) * m_deltaY;
+            const double xc = (xl + xr) / 2, yc = (yl + yr) / 2;
+            double ex = 0;
+            for (int gi = 0; gi < 4; ++gi)
+                for (int gj = 0; gj < 4; ++gj) {
+                    const double gx = xc + 0.5 * m_deltaX * GLP[gi];
+                    const double gy = yc + 0.5 * m_deltaY * GLP[gj];
+                    const double wx = 0.5 * m_deltaX * GLWT[gi];
+                    const double wy = 0.5 * m_deltaY * GLWT[gj];
+                    ex += equation->theVarExact(gx, gy, now) * wx * wy / m_deltaX / m_deltaY;
+                }
+            const size_t c = (size_t)ej * nx + ei;
+            const double u4[4] = {m_Uh[c], m_Uh[nc + c], m_Uh[2 * nc + c], m_Uh[3 * nc + c]};
+            const double e = std::fabs(ex - equation->theVarUh(u4));
+            err1 += e;
+            err2 += std::pow(e, 2);
+            if (e > errinf) errinf = e;
+        }
+    err1 = err1 / nx / ny;
+    err2 = std::sqrt(err2 / nx / ny);
+    std::filesystem::create_directories(m_outputDir);
+    const std::string fn = m_outputDir + "accuracy_" + prefix + ".csv";
+    const bool exists = std::filesystem::exists(fn);
+    std::ofstream f(fn, std::ios::app);
+    if (!exists) f << "Cell Number, Linf-norm, L1-norm, L2-norm" << std::endl;
+    f << nx << ", " << std::setprecision(15) << std::setw(20) << std::scientific << errinf << ", " << std::setw(20)
+      << err1 << ", " << std::setw(20) << err2 << std::endl;
+    return {errinf, err1, err2};
+}
+
+void CWENOFV::writeState(const std::string &path) const {
+    std::ofstream f(path, std::ios::binary);
+    const int hdr[4] = {0x57464f56, 4, m_worldElemNumX, m_worldElemNumY};
+    f.write(reinterpret_cast<const char *>(hdr), sizeof hdr);
+    f.write(reinterpret_cast<const char *>(m_Uh.data()), m_Uh.size() * sizeof(double));
+}
+
+void CWENOFV::readState(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    int hdr[4];
+    f.read(reinterpret_cast<char *>(hdr), sizeof hdr);
+    if (!f || hdr[0] != 0x57464f56 || hdr[2] != m_worldElemNumX || hdr[3] != m_worldElemNumY)
+        throw std::runtime_error("readState: " + path + " is not a state of this grid");
+    m_Uh.resize(4 * (size_t)m_worldElemNumX * m_worldElemNumY);
+    f.read(reinterpret_cast<char *>(m_Uh.data()), m_Uh.size() * sizeof(double));
+    check(wfv_set_state(ctx_, m_Uh.data()), "readState");
+}
+
+void CWENOFV::run() {
+    const auto t0 = std::chrono::steady_clock::now();
+    initializeSolver();
+    double now = 0;
+    int count = 0;
+    if (m_testcase == SMOOTH || m_testcase == VORTEX) outputAccuracy("initial", now);
+    const auto t1 = std::chrono::steady_clock::now();
+    // loop over the output times (src/WENOFV.cpp:318-364); the steps between two output
+    // times run on the device (wfv_advance replays the clipped CFL steps from a graph)
+    for (size_t k = 1; k < m_timeMoments.size(); ++k) {
+        int n = 0;
+        double t = now;
+        check(wfv_advance(ctx_, now, m_timeMoments[k], 0, &n, &t, nullptr, 0), "run");
+        now = t;
+        count += n;
+        std::cout << "t = " << now << "  steps " << count << std::endl;
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    unsigned flags = 0;
+    check(wfv_get_flags(ctx_, &flags), "flags");
+    if (flags & WFV_FLAG_NAN_FLUX) std::cerr << "Riemann flux is nan." << std::endl;
+    if (flags & WFV_FLAG_BAD_THETA) std::cerr << "Error: illegal theta in PP limiting..." << std::endl;
+    std::cout << "\nTotal iterations completed. Total iterations: " << count << std::endl;
+    if (m_testcase == SMOOTH || m_testcase == VORTEX || m_testcase == SHARP) {
+        const auto e = outputAccuracy("final", now);
+        std::cout << std::setprecision(15) << std::scientific << "accuracy (Linf, L1, L2): " << e[0] << " " << e[1]
+                  << " " << e[2] << std::endl;
+    } else {
+        getWorldUh();
+    }
+    std::filesystem::create_directories(m_outputDir);
+    writeState(m_outputDir + "final_state.bin");
+    const double loop = std::chrono::duration<double>(t2 - t1).count();
+    const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double cs = 3.0 * count * m_worldElemNumX * m_worldElemNumY;
+    std::cout << std::defaultfloat << "time loop " << loop << " s (" << cs / loop / 1e6
+              << " M cell-stage/s for RK3), total " << total << " s" << std::endl;
+}
+
+extern "C" int wfv_reference_case(int testcase, wfv_case *cs, int *boundary, double *t_end) {
+    if (!cs || testcase < 0 || testcase > 8) return WFV_ERR_ARG;
+    TwoPhaseEquation e;
+    e.setEquationParameters((TESTCASETYPE)testcase);
+    *cs = wfv_case{e.xL, e.xR, e.yL, e.yR, e.g_rho10, e.g_rho20, e.g_sound1, e.g_sound2, e.g_preb, e.g_gravity};
+    if (boundary) *boundary = e.boundaryCondition == PERIOD ? 0 : 1;
+    if (t_end) *t_end = e.outputTimes.back();
+    return WFV_OK;
+}
+
+extern "C" int wfv_reference_initial_state(int testcase, int nx, int ny, int geometry_ranks, double *U) {
+    if (!U || testcase < 0 || testcase > 8 || nx < 1 || ny < 1) return WFV_ERR_ARG;
+    TwoPhaseEquation e;
+    e.setEquationParameters((TESTCASETYPE)testcase);
+    const size_t nc = (size_t)nx * ny;
+    std::vector<double> dxC(nx), dxW(nx), dyC(ny), xc(nx), yc(ny);
+    wfv::grid_widths(nx, ny, e.xL, e.xR, e.yL, e.yR, geometry_ranks, dxC.data(), dxW.data(), dyC.data(), xc.data(),
+                     yc.data());
+    double u[4];
+    for (int ei = 0; ei < nx; ++ei)
+        for (int ej = 0; ej < ny; ++ej) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int gi = 0; gi < 4; ++gi)
+                for (int gj = 0; gj < 4; ++gj) {
+                    const double gx = xc[ei] + 0.5 * dxC[ei] * GLP[gi];
+                    const double gy = yc[ej] + 0.5 * dyC[ej] * GLP[gj];
+                    const double wx = 0.5 * dxC[ei] * GLWT[gi];
+                    const double wy = 0.5 * dyC[ej] * GLWT[gj];
+                    e.getU0(gx, gy, u);
+                    for (int r = 0; r < 4; ++r) acc[r] += u[r] * wx * wy;
+                }
+            const double area = dxC[ei] * dyC[ej];
+            for (int r = 0; r < 4; ++r) U[r * nc + (size_t)ej * nx + ei] = acc[r] / area;
+        }
+    return WFV_OK;
+}

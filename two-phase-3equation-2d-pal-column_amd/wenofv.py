@@ -262,3 +262,26 @@ class Batch:
         c = ctypes.c_longlong()
         self._check(_batch_lib().wfv_batch_cells(self._h, ctypes.byref(c)))
         return c.value
+
+
+def reference_case(testcase):
+    """Parameters of a reference test case (TESTCASE 0..8): dict for Solver(**...), the
+    boundary and the final output time (host only)."""
+    L = lib()
+    L.wfv_reference_case.argtypes = [ctypes.c_int, ctypes.POINTER(Case), ctypes.POINTER(ctypes.c_int), _dp]
+    c, bc, t = Case(), ctypes.c_int(), ctypes.c_double()
+    if L.wfv_reference_case(testcase, ctypes.byref(c), ctypes.byref(bc), ctypes.byref(t)) != OK:
+        raise WenofvError(f"unknown test case {testcase}")
+    kw = dict(domain=(c.xL, c.xR, c.yL, c.yR), rho10=c.rho10, rho20=c.rho20, sound1=c.sound1, sound2=c.sound2,
+              preb=c.preb, gravity=c.gravity, bc=bc.value)
+    return kw, t.value
+
+
+def reference_initial_state(testcase, nx, ny, geometry_ranks=4):
+    """Initial cell averages of a reference test case as CWENOFV::initializeAve builds them."""
+    L = lib()
+    L.wfv_reference_initial_state.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    U = np.empty((4, ny, nx))
+    if L.wfv_reference_initial_state(testcase, nx, ny, geometry_ranks, _p(U)) != OK:
+        raise WenofvError("bad arguments")
+    return U
