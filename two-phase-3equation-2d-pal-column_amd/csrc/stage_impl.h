@@ -190,14 +190,14 @@ WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceS
     using F = typename PR::F;
 #pragma unroll
     for (int v = 0; v < 4; ++v) G[v] = 0.0;
-#pragma unroll 1
-    for (int g = 0; g < 4; ++g) {
+    // node in register slot `slot` with quadrature weight GLW[g] width / 2
+    auto node = [&](int slot, int g) {
         double ul[4], ur[4];
         F fl[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            ul[v] = a.get(v, 0, ba[v]);
-            ur[v] = b.get(v, 0, bb[v]);
+            ul[v] = a.get(v, slot, ba[v]);
+            ur[v] = b.get(v, slot, bb[v]);
         }
         double pref;
         riemann<F, FLUX, NX, NY>(P, ul, ur, fl, pref);
@@ -212,8 +212,17 @@ WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceS
                 G[v] += fv * qw;
             }
         }
-        a.rotate();  // node g+1 moves to slot 0
-        b.rotate();
+    };
+    if constexpr (PR::exact) {
+#pragma unroll 1
+        for (int g = 0; g < 4; ++g) {
+            node(0, g);
+            a.rotate();  // node g+1 moves to slot 0
+            b.rotate();
+        }
+    } else {  // the mixed solver is small enough to unroll without register rotation
+#pragma unroll
+        for (int g = 0; g < 4; ++g) node(g, g);
     }
 }
 
@@ -256,6 +265,11 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     const double cdt = A.c * dt;
     const double dxc = owned ? A.dxC[cx] : 1.0;
     const double dxw = owned ? A.dxW[cx] : 1.0;
+    // mixed policy: reciprocals of the widths replace the reference's divisions by them
+    const double idxc = EX ? 0.0 : 1.0 / dxc;
+    const double idxw = EX ? 0.0 : 1.0 / dxw;
+    const double idX = EX ? 0.0 : 1.0 / A.dX;
+    const double idY = EX ? 0.0 : 1.0 / A.dY;
     constexpr double omega1 = 0.1;
     constexpr double omegaR = 1.0 - 2 * omega1;
 
@@ -476,12 +490,23 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 // getPositivityPreservingTheta (src/WENOFV.cpp:1936-2000), double in both policies
                 Eos e;
                 eos_p<!EX>(P, Uc, e);
-                const double u = Uc[2] / (Uc[0] + Uc[1]);
-                const double v = Uc[3] / (Uc[0] + Uc[1]);
-                const double a1 = fabs(u * 1.0 + v * 0.0) + e.c;
-                const double a2 = fabs(u * 0.0 + v * 1.0) + e.c;
-                const double mu1 = (a1 / A.dX) / (a1 / A.dX + a2 / A.dY);
-                const double mu2 = (a2 / A.dY) / (a1 / A.dX + a2 / A.dY);
+                double mu1, mu2;
+                if constexpr (EX) {
+                    const double u = Uc[2] / (Uc[0] + Uc[1]);
+                    const double v = Uc[3] / (Uc[0] + Uc[1]);
+                    const double a1 = fabs(u * 1.0 + v * 0.0) + e.c;
+                    const double a2 = fabs(u * 0.0 + v * 1.0) + e.c;
+                    mu1 = (a1 / A.dX) / (a1 / A.dX + a2 / A.dY);
+                    mu2 = (a2 / A.dY) / (a1 / A.dX + a2 / A.dY);
+                } else {
+                    const double irho = rcp_nr(Uc[0] + Uc[1]);
+                    const double a1 = fabs(Uc[2] * irho) + e.c;
+                    const double a2 = fabs(Uc[3] * irho) + e.c;
+                    const double t1 = a1 * idX, t2 = a2 * idY;
+                    const double it = rcp_nr(t1 + t2);
+                    mu1 = t1 * it;
+                    mu2 = t2 * it;
+                }
                 double th[2];
 #pragma unroll
                 for (int k = 0; k < 2; ++k) {
@@ -496,7 +521,8 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                         rem -= cg * mu1 * fw.get(k, g, uave);
                         rem -= cg * mu1 * fe.get(k, g, uave);
                     }
-                    rem /= omegaR;
+                    if constexpr (EX) rem /= omegaR;
+                    else rem *= 1.0 / omegaR;
                     double mloc = rem;
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
@@ -552,7 +578,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 face_flux<PR, FLUX, 0, 1>(P, top_prev, top_base, fs, Uc, dxc, G, err, true);
             }
             double rhs_x[4] = {0.0, 0.0, 0.0, 0.0};
-            double dyr = 1.0;
+            double dyr = 1.0, idyr = 0.0;
             if (interior_row) {
                 // ---- 4. face east of cell (xc, r): right face | neighbour lane's left face
                 wave_sync();  // every lane has read the y-sweep values the west faces overwrite
@@ -562,6 +588,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
                 wave_sync();
                 dyr = A.dyC[r];
+                if constexpr (!EX) idyr = 1.0 / dyr;
                 if (xface_lane) {
                     // the neighbour's west face values, as exact FP64 states (base 0)
                     FS nb;  // the neighbour's west face values; momenta relative to its node 0
@@ -582,10 +609,14 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     // RHS order of assembleBoundaryFaceTerm: +west, -east (src/WENOFV.cpp:1802-1813)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        double acc = 0.0;
-                        acc += L.fx[v][lane - 1] / dxw / dyr;
-                        acc += -L.fx[v][lane] / dxc / dyr;
-                        rhs_x[v] = acc;
+                        if constexpr (EX) {
+                            double acc = 0.0;
+                            acc += L.fx[v][lane - 1] / dxw / dyr;
+                            acc += -L.fx[v][lane] / dxc / dyr;
+                            rhs_x[v] = acc;
+                        } else {
+                            rhs_x[v] = (L.fx[v][lane - 1] * idxw - L.fx[v][lane] * idxc) * idyr;
+                        }
                     }
                 }
             }
@@ -594,6 +625,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             if (it >= 2 && owned) {
                 const int rr = r - 1;
                 const double dyp = A.dyC[rr];
+                const double idyp = EX ? 0.0 : 1.0 / dyp;
                 const double *Ucur = win[1];
                 double Ub[4];
                 if (A.Un) {
@@ -606,7 +638,8 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     double rhs = rhs_part[v];
-                    rhs -= G[v] / dxc / dyp;
+                    if constexpr (EX) rhs -= G[v] / dxc / dyp;
+                    else rhs -= G[v] * idxc * idyp;
                     if (v == 3 && A.gravity != 0.0) rhs -= (Ucur[0] + Ucur[1]) * A.gravity;
                     Unew[v] = A.a * Ub[v] + A.b * Ucur[v] + cdt * rhs;
                     A.Uo[v * nc + (size_t)rr * nx + xc] = Unew[v];
@@ -615,18 +648,27 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     // calculateDeltaT on the new state (src/WENOFV.cpp:489-513)
                     Eos e;
                     eos_p<!EX>(P, Unew, e);
-                    const double u = Unew[2] / (Unew[0] + Unew[1]);
-                    const double v = Unew[3] / (Unew[0] + Unew[1]);
-                    const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
-                    const double ey = fabs(u * 0.0 + v * 1.0) + e.c;
-                    const double tmp = A.cfl / (ex / A.dX + ey / A.dY);
+                    double tmp;
+                    if constexpr (EX) {
+                        const double u = Unew[2] / (Unew[0] + Unew[1]);
+                        const double v = Unew[3] / (Unew[0] + Unew[1]);
+                        const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
+                        const double ey = fabs(u * 0.0 + v * 1.0) + e.c;
+                        tmp = A.cfl / (ex / A.dX + ey / A.dY);
+                    } else {
+                        const double irho = rcp_nr(Unew[0] + Unew[1]);
+                        tmp = A.cfl / ((fabs(Unew[2] * irho) + e.c) * idX + (fabs(Unew[3] * irho) + e.c) * idY);
+                    }
                     if (isnan(tmp)) err |= WFV_FLAG_NAN_DT;
                     else dtloc = fmin(dtloc, tmp);
                 }
             }
             if (interior_row && owned) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) rhs_part[v] = rhs_x[v] + G[v] / dxc / dyr;
+                for (int v = 0; v < 4; ++v) {
+                    if constexpr (EX) rhs_part[v] = rhs_x[v] + G[v] / dxc / dyr;
+                    else rhs_part[v] = rhs_x[v] + G[v] * (idxc * idyr);
+                }
             }
         }
 #pragma unroll

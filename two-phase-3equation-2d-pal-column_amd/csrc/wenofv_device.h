@@ -52,6 +52,16 @@ WFV_DEV float stdMin(float a, float b) { return (b < a) ? b : a; }
 WFV_DEV float stdMax(float a, float b) { return (a < b) ? b : a; }
 WFV_DEV double scMin(double a, double b) { return a < b ? a : b; }      // sc_math::Min
 
+// min/max/sqrt of a policy's arithmetic type: std::min/std::max and IEEE sqrt in double; the
+// single hardware instructions in float (they differ only for NaN operands and in the last
+// bit of the square root)
+WFV_DEV double wMin(double a, double b) { return stdMin(a, b); }
+WFV_DEV double wMax(double a, double b) { return stdMax(a, b); }
+WFV_DEV float wMin(float a, float b) { return __builtin_fminf(a, b); }
+WFV_DEV float wMax(float a, float b) { return __builtin_fmaxf(a, b); }
+WFV_DEV double wSqrt(double x) { return sqrt(x); }
+WFV_DEV float wSqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // ------------------------------------------------------------------ equation of state
 // TwoPhaseEquation::getVolumeFrac / getPressure / getSoundSpeed (include/Equations.hpp:84-146)
 // evaluated once per state and shared by every use, in double in both policies.
@@ -88,24 +98,24 @@ WFV_DEV double rsq_nr(double x) {
     return fma(y, fma(-h, y, 0.5), y);
 }
 
-// The same equation of state with the algebra rearranged to one square root and three
-// reciprocals (mixed policy; still double): with N = dq + S and D = 2 z2rho2 c2^2,
-// z1 = N/(D+N), z2 = D/(D+N), rho2 = (D+N)/(2 c2^2), rho1 = z1rho1 (D+N)/N, and
-// 1/c^2 = rho (z1^2/(z1rho1 c1^2) + z2^2/(z2rho2 c2^2)).
+WFV_DEV double sqrt_nr(double x) { return x > 0.0 ? x * rsq_nr(x) : 0.0; }
+
+// Mixed policy: pressure and sound speed (still double) with the algebra rearranged to no
+// division.  With S the square root of the volume-fraction quadratic's discriminant,
+// N = dq + S and D = 2 z2rho2 c2^2: z1 = N/(D+N), z2 = D/(D+N) and rho2 = (D+N)/(2 c2^2),
+// so p = c2^2 (rho2 - rho20) = (D+N)/2 - c2^2 rho20 (equal to c1^2 (rho1 - rho10) by the
+// pressure equilibrium the quadratic expresses), and Wood's
+// 1/c^2 = rho (z1^2/(z1rho1 c1^2) + z2^2/(z2rho2 c2^2)) = rho (N^2/(z1rho1 c1^2) + 2D)/(D+N)^2.
+// Only rho, p and c are set.
 WFV_DEV void eos_fast(const Phys &P, const double U[4], Eos &e) {
     const double a = U[0], b = U[1];
     const double dq = P.q - (b * P.c2_2 - a * P.c1_2);
-    const double S = sqrt(dq * dq + 4.0 * a * P.c1_2 * b * P.c2_2);
+    const double S = sqrt_nr(dq * dq + 4.0 * a * P.c1_2 * b * P.c2_2);
     const double N = dq + S, D = 2.0 * b * P.c2_2;
-    const double inv = rcp_nr(D + N);
-    const double z1 = N * inv, z2 = D * inv;
+    const double DN = D + N;
     e.rho = a + b;
-    e.z1 = z1;
-    const double rho2 = (D + N) * (0.5 / P.c2_2);
-    const double rho1 = a * (D + N) * rcp_nr(N);
-    e.p = (z1 < z2) ? P.c2_2 * (rho2 - P.rho20) : P.c1_2 * (rho1 - P.rho10);
-    const double ic2 = e.rho * (z1 * z1 * rcp_nr(a * P.c1_2) + z2 * z2 * rcp_nr(b * P.c2_2));
-    e.c = rsq_nr(ic2);
+    e.p = 0.5 * DN - P.c2_2 * P.rho20;
+    e.c = DN * rsq_nr(e.rho * (N * N * rcp_nr(a * P.c1_2) + 2.0 * D));
 }
 
 template <bool FAST>
@@ -150,7 +160,7 @@ WFV_DEV void riemann_llf(const Phys &P, const double dl[4], const double dr[4], 
     const T ur = dv(UR[2], rr), vr = dv(UR[3], rr);
     const T wsL = fabs(ul * nx + vl * ny) + T(el.c);
     const T wsR = fabs(ur * nx + vr * ny) + T(er.c);
-    const T ws = stdMax(wsL, wsR);
+    const T ws = wMax(wsL, wsR);
     T FL[4], FR[4];
     phy_flux<T, NX, NY>(UL, rl, ul, vl, T(el.p - pref), FL);
     phy_flux<T, NX, NY>(UR, rr, ur, vr, T(er.p - pref), FR);
@@ -183,13 +193,13 @@ WFV_DEV void riemann_hllc(const Phys &P, const double dl[4], const double dr[4],
     FL[2] = UL[2] * vn_L + pre_L * nx; FL[3] = UL[3] * vn_L + pre_L * ny;
     FR[0] = UR[0] * vn_R; FR[1] = UR[1] * vn_R;
     FR[2] = UR[2] * vn_R + pre_R * nx; FR[3] = UR[3] * vn_R + pre_R * ny;
-    const T sL = sqrt(rho_L), sR = sqrt(rho_R);
+    const T sL = wSqrt(rho_L), sR = wSqrt(rho_R);
     const T vn_roe = dv(vn_L * sL + vn_R * sR, sL + sR);
     T c2_roe = dv(c_L * c_L * sL + c_R * c_R * sR, sL + sR);
     c2_roe += dv(dv(T(0.5) * sL * sR, sL + sR), sL + sR) * (vn_R - vn_L) * (vn_R - vn_L);
-    const T c_roe = sqrt(c2_roe);
-    const T S_L = stdMin(vn_L - c_L, vn_roe - c_roe);
-    const T S_R = stdMax(vn_R + c_R, vn_roe + c_roe);
+    const T c_roe = wSqrt(c2_roe);
+    const T S_L = wMin(vn_L - c_L, vn_roe - c_roe);
+    const T S_R = wMax(vn_R + c_R, vn_roe + c_roe);
     // with pref = p_L the shifted pressures already form the jump pre_R - pre_L
     const T S_star = dv((S_L - vn_L) * rho_L * vn_L - (S_R - vn_R) * rho_R * vn_R + pre_R - pre_L,
                         (S_L - vn_L) * rho_L - (S_R - vn_R) * rho_R);
@@ -229,7 +239,7 @@ WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], T F[
 template <class T, bool XDIR>
 WFV_DEV void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
     Eos e;
-    eos_p<sizeof(T) == 4>(P, U, e);
+    eos(P, U, e);
     const double rho = e.rho, u = U[2] / rho, v = U[3] / rho, c = e.c;
     const double z1 = e.z1, z2 = 1.0 - z1;
     const double zrho1 = U[0] * P.c1_2 / (z1 * z1);
@@ -320,8 +330,8 @@ struct Weno5 {
         } else {
             // WENO5Zthreconstruction (include/Utility.hpp:261-280): beta of the min-max
             // normalised stencil, tau = |beta0 - beta2|, eps = 1e-10
-            const T mx = stdMax(stdMax(stdMax(a, b), stdMax(e, d)), c);
-            const T mn = stdMin(stdMin(stdMin(a, b), stdMin(e, d)), c);
+            const T mx = wMax(wMax(wMax(a, b), wMax(e, d)), c);
+            const T mn = wMin(wMin(wMin(a, b), wMin(e, d)), c);
             const T den = mx - mn + T(1e-20);
             T a_, b_, c_, d_, e_;
             if constexpr (sizeof(T) == 8) {
