@@ -140,6 +140,7 @@ def plain_fp64_per_instance(torch, wenofv, wls, opts, stream, steps):
         s = wenofv.Solver(w.nx, w.ny, **w.solver_kwargs(**dict(opts, precision=0)))
         s.set_stream(stream.cuda_stream)
         s.set_state(w.U0)
+        s.autotune(trials=2)  # rows per strip for this grid (cached per grid size)
         s.advance(fixed_steps=32)
         ms, (n, _, _) = timed(torch, s, stream, steps)
         assert n == steps, (n, steps)
@@ -166,12 +167,15 @@ def run_batch(a, torch, wenofv, cases):
     runs = [("mixed", 1, 0.0)] + ([] if a.no_fp64 else [("fp64", 0, 0.0), ("fp64_p13", 0, 1e-13), ("fp64_p8", 0, 6e-8)])
     if a.no_fp64 and a.precision == 0:  # time the all-FP64 batch alone
         runs = [("mixed", 0, 0.0)]
+    tiles = {}
     for tag, prec, perturb in runs:
         b = wenofv.Batch(insts, rows_per_strip=a.ty, **dict(opts, precision=prec))
         b.set_stream(stream.cuda_stream)
+        for k, w in enumerate(wls):
+            b.set_state(k, w.U0)
+        if not a.ty:  # rows per strip per grid class and the schedule from the autotuner
+            tiles[prec] = b.autotune(trials=2, cache=False)
         if a.warmup:
-            for k, w in enumerate(wls):
-                b.set_state(k, w.U0)
             b.advance(fixed_steps=max(a.warmup, 16))  # also captures the step graph
         for k, w in enumerate(wls):
             b.set_state(k, w.U0 * (1 + perturb * rng.standard_normal(w.U0.shape)) if perturb else w.U0)
@@ -187,7 +191,8 @@ def run_batch(a, torch, wenofv, cases):
         "data": "synthetic (seeded initial conditions, no dataset)",
         "config": {"workload": f"batch256: 256 liquid-column instances (seed 4), grids {{64,128,256,512}}^2, walls, "
                                f"mixed WENO-Z HLLC BP RK3 CFL0.4, {steps} steps each, one launch per stage",
-                   "instances": len(wls), "cells": cells, "flags": results["mixed"]["flags"]},
+                   "instances": len(wls), "cells": cells, "flags": results["mixed"]["flags"],
+                   "tiles": tiles.get(runs[0][1], {"rows_per_strip": a.ty})},
         "roofline": roofline(cells, results["mixed"]["ms"] / (3 * steps),
                              profiled_traffic(f"batch_stage_{'fp64' if runs[0][1] == 0 else 'mixed'}_batch256.md")),
     }
@@ -265,6 +270,7 @@ def run_single(a, torch, wenofv, cases):
     stream = torch.cuda.Stream()
     sol.set_stream(stream.cuda_stream)
     sol.set_state(wl.U0)
+    tiles = sol.autotune(trials=2, cache=False) if not a.ty else {"rows_per_strip": a.ty}
     if a.warmup:
         sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
     sol.set_state(wl.U0)
@@ -280,7 +286,7 @@ def run_single(a, torch, wenofv, cases):
         "data": "synthetic (seeded initial condition, no dataset)",
         "config": {"workload": f"{wl.name} {wl.nx}x{wl.ny} walls WENO-Z HLLC BP RK3 CFL0.4",
                    "precision": "fp64" if prec == 0 else "mixed", "cells": cells, "final_time": t,
-                   "flags": sol.flags(), "finite": bool(np.isfinite(U).all())},
+                   "flags": sol.flags(), "finite": bool(np.isfinite(U).all()), "tiles": tiles},
         "roofline": roofline(cells, ms / (3 * steps)),
     }
     if not a.no_cpu:

@@ -131,9 +131,31 @@ int wfv_batch_get_state(wfv_batch *b, int i, double *U);
 int wfv_batch_advance(wfv_batch *b, double t_end, int fixed_steps, int *nsteps, double *t_out,
                       unsigned *flags);
 int wfv_batch_set_stream(wfv_batch *b, void *hip_stream);
-/* rows per wavefront strip of instance i (the autotuner's choice; 0 keeps the default) */
+/* rows per wavefront strip of instance i (0: the batch default) */
 int wfv_batch_set_rows_per_strip(wfv_batch *b, int i, int rows);
+int wfv_batch_get_rows_per_strip(const wfv_batch *b, int i, int *rows);
+/* schedule of the stage launches: 0 one wavefront per (instance, strip) item, 1 a
+   persistent grid of 8 wavefronts per CU striding over the items */
+int wfv_batch_set_schedule(wfv_batch *b, int persistent);
+int wfv_batch_get_schedule(const wfv_batch *b, int *persistent);
 int wfv_batch_cells(const wfv_batch *b, long long *cells);  /* total interior cells */
+
+/* ---------------------------------------------------------------------------------------
+   Launch-shape autotuner.  No reference counterpart (the reference's CPU loops have no
+   tile shape).  Times `trials` launches of the first RK stage on the current state (which
+   it leaves unchanged) for each candidate rows-per-strip (8..128) and, for a batch, each
+   schedule, and keeps the fastest; a batch is tuned per grid-size class (instances of equal
+   nx x ny) by coordinate descent.  Results are also settable directly (a cache of earlier
+   tunings): wfv_set_rows_per_strip, wfv_batch_set_rows_per_strip, wfv_batch_set_schedule. */
+typedef struct wfv_tuning {
+    int rows_per_strip;  /* single context: the choice (batches: per instance, see above) */
+    int persistent;      /* batch schedule chosen */
+    double stage_ms;     /* mean duration of one stage launch with the choice */
+} wfv_tuning;
+int wfv_autotune(wfv_ctx *ctx, int trials, wfv_tuning *best);
+int wfv_batch_autotune(wfv_batch *b, int trials, wfv_tuning *best);
+int wfv_set_rows_per_strip(wfv_ctx *ctx, int rows);
+int wfv_get_rows_per_strip(const wfv_ctx *ctx, int *rows);
 
 /* Grid of the context: interior cells and the per-column / per-row widths. */
 int wfv_grid(const wfv_ctx *ctx, int *nx, int *ny, double *dxC, double *dxW, double *dyC);

@@ -26,6 +26,7 @@ UNITS = [
     ("wenofv_kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("wenofv_control.hip", ["-ffp-contract=off"]),
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),
+    ("wenofv_tune.cpp", ["-ffp-contract=off"]),
     ("WENOFV.cpp", ["-ffp-contract=off"]),
 ]
 CLI = os.path.join(HERE, "wenofv_run")

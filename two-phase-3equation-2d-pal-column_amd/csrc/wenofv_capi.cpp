@@ -15,47 +15,27 @@
 
 #include "wenofv.h"
 #include "wenofv_geometry.h"
+#include "wenofv_host.h"
 #include "wenofv_kernels.h"
 
 using namespace wfv;
 
-struct wfv_ctx {
-    wfv_options opt{};
-    wfv_case cs{};
-    int nx = 0, ny = 0;
-    size_t nc = 0;
-    double *reg[3] = {nullptr, nullptr, nullptr};  // RK registers; reg[0] is the state
-    double *d_dxC = nullptr, *d_dxW = nullptr, *d_dyC = nullptr;
-    double *d_faces = nullptr, *d_theta = nullptr;
-    StepCtl *d_ctl = nullptr;
-    double *d_dts = nullptr;
-    int dts_cap = 0;
-    hipStream_t stream = nullptr;     // stream in use
-    hipStream_t own_stream = nullptr; // stream created with the context
-    Phys P{};
-    double dX = 0, dY = 0;
-    int ty = 16;
-    std::vector<double> dxC, dxW, dyC;
-    std::string err;
-    hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
-};
 
 namespace {
-
 thread_local std::string g_create_error;  // reported by wfv_last_error(NULL)
+}  // namespace
+
+namespace wfv {
 
 int fail(wfv_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
 }
 
-#define HIPCHK(ctx, call)                                                                    \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(ctx, WFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+int bfail(wfv_batch *b, int code, const std::string &msg) {
+    if (b) b->err = msg;
+    return code;
+}
 
 StageArgs base_args(wfv_ctx *c) {
     StageArgs A{};
@@ -79,11 +59,6 @@ StageArgs base_args(wfv_ctx *c) {
     return A;
 }
 
-// RK schedule of SSP-RK1/2/3 (src/WENOFV.cpp:376-473): per stage (a, b, c) and registers
-struct StagePlan {
-    double a, b, c;
-    int in, base, out;  // register indices; base < 0: Un = U
-};
 int rk_plan(int rk, StagePlan *s) {
     if (rk == 0) {
         // U += dt*rhs, written as (0*U + 1*U) + (1*dt)*rhs, which rounds identically
@@ -100,6 +75,10 @@ int rk_plan(int rk, StagePlan *s) {
     s[2] = {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 2, 0, 0};
     return 3;
 }
+
+}  // namespace wfv
+
+namespace {
 
 // enqueue one RK step; dt from the host (dt_dev == nullptr) or from the device loop
 int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active, bool want_dtmin) {
@@ -208,16 +187,16 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
 
 void wfv_destroy(wfv_ctx *c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
-    if (c->graph) hipGraphExecDestroy(c->graph);
-    for (auto &p : c->reg) hipFree(p);
-    hipFree(c->d_dxC);
-    hipFree(c->d_dxW);
-    hipFree(c->d_dyC);
-    hipFree(c->d_faces);
-    hipFree(c->d_theta);
-    hipFree(c->d_ctl);
-    hipFree(c->d_dts);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    for (auto &p : c->reg) (void)hipFree(p);
+    (void)hipFree(c->d_dxC);
+    (void)hipFree(c->d_dxW);
+    (void)hipFree(c->d_dyC);
+    (void)hipFree(c->d_faces);
+    (void)hipFree(c->d_theta);
+    (void)hipFree(c->d_ctl);
+    (void)hipFree(c->d_dts);
     if (c->own_stream) {
         (void)hipStreamSynchronize(c->own_stream);
         (void)hipStreamDestroy(c->own_stream);
@@ -308,7 +287,7 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
                 double *dts, int max_dts) {
     if (!c) return WFV_ERR_ARG;
     if (dts && max_dts > c->dts_cap) {
-        hipFree(c->d_dts);
+        (void)hipFree(c->d_dts);
         c->d_dts = nullptr;
         HIPCHK(c, hipMalloc(&c->d_dts, (size_t)max_dts * sizeof(double)));
         c->dts_cap = max_dts;
@@ -416,44 +395,17 @@ int wfv_grid(const wfv_ctx *c, int *nx, int *ny, double *dxC, double *dxW, doubl
 
 // ======================================================================= batches
 
-struct wfv_batch {
-    wfv_options opt{};
-    int n = 0;
-    std::vector<wfv_instance> inst;
-    std::vector<BatchInst> hinst;   // host copy of the instance table
-    std::vector<size_t> cell_off;   // first cell of each instance in the registers
-    std::vector<int> ty;            // rows per strip per instance
-    size_t total_cells = 0;
-    int total_items = 0;
-    int max_cells = 0;
-    double *reg[3] = {nullptr, nullptr, nullptr};
-    double *geo = nullptr;          // all instances' dxC, dxW, dyC
-    StepCtl *d_ctl = nullptr;
-    BatchInst *d_inst = nullptr;
-    int *d_item_inst = nullptr;
-    hipStream_t stream = nullptr, own_stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    int grid = 0;                   // persistent wavefronts per stage launch
-    std::string err;
-};
+namespace wfv {
 
-namespace {
-
-int bfail(wfv_batch *b, int code, const std::string &msg) {
-    if (b) b->err = msg;
-    return code;
-}
-#define BHIPCHK(b, call)                                                                      \
-    do {                                                                                      \
-        hipError_t e_ = (call);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return bfail(b, WFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// (re)build the instance table and the item -> instance map from b->ty
+// (re)build the instance table and the item -> instance map from b->ty.  Items are issued
+// longest first (instances with more rows per strip first), so the short items fill the
+// tail of the launch.
 int batch_layout(wfv_batch *b) {
+    std::vector<int> order(b->n);
+    for (int i = 0; i < b->n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return b->ty[p] > b->ty[q]; });
     std::vector<int> item_inst;
-    for (int i = 0; i < b->n; ++i) {
+    for (int i : order) {
         BatchInst &I = b->hinst[i];
         I.ty = b->ty[i];
         I.strips_x = (I.nx + strip_width() - 1) / strip_width();
@@ -470,16 +422,19 @@ int batch_layout(wfv_batch *b) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    // one wavefront per item by default; WENOFV_BATCH_PERSIST=1 selects the persistent
-    // form (8 wavefronts per CU striding over the items)
-    const char *pe = getenv("WENOFV_BATCH_PERSIST");
-    b->grid = (pe && pe[0] == '1') ? std::min(b->total_items, cus * 8) : b->total_items;
+    // schedule 0: one wavefront per item; 1: persistent, 8 wavefronts per CU striding over
+    // the items
+    b->grid = b->persistent ? std::min(b->total_items, cus * 8) : b->total_items;
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
     return WFV_OK;
 }
+
+}  // namespace wfv
+
+namespace {
 
 int batch_enqueue_step(wfv_batch *b) {
     StagePlan pl[3];
@@ -544,13 +499,20 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     b->cell_off.resize(n);
     b->ty.resize(n);
     size_t geo_total = 0;
+    std::vector<int> nxs(n), nys(n);
+    for (int i = 0; i < n; ++i) {
+        nxs[i] = inst[i].nx;
+        nys[i] = inst[i].ny;
+    }
+    const int ty0 = opt->rows_per_strip > 0 ? opt->rows_per_strip : default_batch_rows_per_strip(nxs.data(), nys.data(), n);
     for (int i = 0; i < n; ++i) {
         b->cell_off[i] = b->total_cells;
         b->total_cells += (size_t)inst[i].nx * inst[i].ny;
         geo_total += 2 * (size_t)inst[i].nx + inst[i].ny;
         b->max_cells = std::max(b->max_cells, inst[i].nx * inst[i].ny);
-        b->ty[i] = opt->rows_per_strip > 0 ? opt->rows_per_strip : default_rows_per_strip(inst[i].nx, inst[i].ny);
+        b->ty[i] = ty0;
     }
+    b->default_ty = ty0;
     auto bad = [&](hipError_t e) {
         if (e == hipSuccess) return false;
         g_create_error = hipGetErrorString(e);
@@ -651,7 +613,7 @@ int wfv_batch_set_stream(wfv_batch *b, void *hip_stream) {
 int wfv_batch_set_rows_per_strip(wfv_batch *b, int i, int rows) {
     if (!b || i < 0 || i >= b->n || rows < 0) return WFV_ERR_ARG;
     BHIPCHK(b, hipStreamSynchronize(b->stream));
-    b->ty[i] = rows > 0 ? rows : default_rows_per_strip(b->inst[i].nx, b->inst[i].ny);
+    b->ty[i] = rows > 0 ? rows : b->default_ty;
     return batch_layout(b);
 }
 

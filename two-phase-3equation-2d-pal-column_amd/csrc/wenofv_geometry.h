@@ -46,4 +46,16 @@ inline int default_rows_per_strip(int nx, int ny) {
     return 8;
 }
 
+// Rows per strip of every instance of a batch: the longest strips that still give the
+// launch at least two rounds of 2048 wavefronts (8 per CU of a 256-CU device).
+inline int default_batch_rows_per_strip(const int *nx, const int *ny, int n) {
+    const int cand[] = {64, 48, 32, 24, 16, 12, 8};
+    for (int ty : cand) {
+        long waves = 0;
+        for (int i = 0; i < n; ++i) waves += (long)((nx[i] + 57) / 58) * ((ny[i] + ty - 1) / ty);
+        if (waves >= 4096) return ty;
+    }
+    return 8;
+}
+
 }  // namespace wfv

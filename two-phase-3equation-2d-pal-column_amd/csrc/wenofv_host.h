@@ -1,0 +1,89 @@
+// wenofv_host.h -- host-side state of solver contexts and batches (library-internal).
+//
+// Shared by the C ABI (wenofv_capi.cpp) and the launch-shape autotuner (wenofv_tune.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "wenofv.h"
+#include "wenofv_kernels.h"
+
+struct wfv_ctx {
+    wfv_options opt{};
+    wfv_case cs{};
+    int nx = 0, ny = 0;
+    size_t nc = 0;
+    double *reg[3] = {nullptr, nullptr, nullptr};  // RK registers; reg[0] is the state
+    double *d_dxC = nullptr, *d_dxW = nullptr, *d_dyC = nullptr;
+    double *d_faces = nullptr, *d_theta = nullptr;
+    wfv::StepCtl *d_ctl = nullptr;
+    double *d_dts = nullptr;
+    int dts_cap = 0;
+    hipStream_t stream = nullptr;     // stream in use
+    hipStream_t own_stream = nullptr; // stream created with the context
+    wfv::Phys P{};
+    double dX = 0, dY = 0;
+    int ty = 16;                      // rows per wavefront strip
+    std::vector<double> dxC, dxW, dyC;
+    std::string err;
+    hipGraphExec_t graph = nullptr;   // captured steps; rebuilt when the launch shape changes
+    int graph_steps = 0;
+};
+
+struct wfv_batch {
+    wfv_options opt{};
+    int n = 0;
+    std::vector<wfv_instance> inst;
+    std::vector<wfv::BatchInst> hinst;  // host copy of the instance table
+    std::vector<size_t> cell_off;       // first cell of each instance in the registers
+    std::vector<int> ty;                // rows per strip per instance
+    int default_ty = 16;                // the batch default of rows per strip
+    size_t total_cells = 0;
+    int total_items = 0;
+    int max_cells = 0;
+    double *reg[3] = {nullptr, nullptr, nullptr};
+    double *geo = nullptr;              // all instances' dxC, dxW, dyC
+    wfv::StepCtl *d_ctl = nullptr;
+    wfv::BatchInst *d_inst = nullptr;
+    int *d_item_inst = nullptr;
+    hipStream_t stream = nullptr, own_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int persistent = 0;                 // schedule: 0 one wavefront per item, 1 persistent grid
+    int grid = 0;                       // wavefronts per stage launch
+    std::string err;
+};
+
+namespace wfv {
+
+// RK schedule of SSP-RK1/2/3 (src/WENOFV.cpp:376-473): per stage (a, b, c) and registers
+struct StagePlan {
+    double a, b, c;
+    int in, base, out;  // register indices; base < 0: Un = U
+};
+int rk_plan(int rk, StagePlan *s);
+
+int fail(wfv_ctx *c, int code, const std::string &msg);
+int bfail(wfv_batch *b, int code, const std::string &msg);
+
+// stage arguments of a context without registers, RK coefficients or dt
+StageArgs base_args(wfv_ctx *c);
+// (re)build a batch's instance table, item list and launch grid from its rows per strip
+// and schedule; drops the captured step graph
+int batch_layout(wfv_batch *b);
+
+}  // namespace wfv
+
+#define HIPCHK(ctx, call)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return wfv::fail(ctx, WFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define BHIPCHK(b, call)                                                                         \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return wfv::bfail(b, WFV_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)

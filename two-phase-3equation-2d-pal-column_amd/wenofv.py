@@ -9,6 +9,7 @@ arrays of shape (4, ny, nx) holding (z1*rho1, z2*rho2, rho*u, rho*v).
 Loading fails loudly when the library has not been built: there is no fallback.
 """
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -35,8 +36,38 @@ class Case(ctypes.Structure):
                 ("sound2", ctypes.c_double), ("preb", ctypes.c_double), ("gravity", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("rows_per_strip", ctypes.c_int), ("persistent", ctypes.c_int), ("stage_ms", ctypes.c_double)]
+
+
 class WenofvError(RuntimeError):
     pass
+
+
+def _cache_path():
+    return os.environ.get("WENOFV_TUNING_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "wenofv_tuning.json"))
+
+
+def _cache_load():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _cache_store(key, value):
+    d = _cache_load()
+    d[key] = value
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache
 
 
 _lib = None
@@ -66,6 +97,9 @@ def lib():
     L.wfv_synchronize.argtypes = [vp]
     L.wfv_set_stream.argtypes = [vp, vp]
     L.wfv_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp]
+    L.wfv_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Tuning)]
+    L.wfv_set_rows_per_strip.argtypes = [vp, ctypes.c_int]
+    L.wfv_get_rows_per_strip.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.wfv_version.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -159,6 +193,36 @@ class Solver:
     def synchronize(self):
         self._check(lib().wfv_synchronize(self._h))
 
+    def _key(self):
+        o = self.opt
+        return (f"solver:{version()}:p{o.precision}r{o.reconstruction}s{o.scheme}f{o.riemann_flux}b{o.pp_limiter}"
+                f":{self.nx}x{self.ny}")
+
+    def autotune(self, trials=2, cache=True):
+        """Choose the rows per wavefront strip by timing stage launches on the current state
+        (left unchanged).  With cache, an earlier choice for the same options and grid is
+        reused.  Returns {"rows_per_strip", "stage_ms"}."""
+        key = self._key()
+        if cache:
+            hit = _cache_load().get(key)
+            if hit:
+                self.set_rows_per_strip(hit["rows_per_strip"])
+                return hit
+        t = Tuning()
+        self._check(lib().wfv_autotune(self._h, trials, ctypes.byref(t)))
+        res = {"rows_per_strip": t.rows_per_strip, "stage_ms": t.stage_ms}
+        if cache:
+            _cache_store(key, res)
+        return res
+
+    def set_rows_per_strip(self, rows):
+        self._check(lib().wfv_set_rows_per_strip(self._h, rows))
+
+    def rows_per_strip(self):
+        r = ctypes.c_int()
+        self._check(lib().wfv_get_rows_per_strip(self._h, ctypes.byref(r)))
+        return r.value
+
     def grid(self):
         nx, ny = ctypes.c_int(), ctypes.c_int()
         dxC, dxW, dyC = np.empty(self.nx), np.empty(self.nx), np.empty(self.ny)
@@ -191,6 +255,10 @@ def _batch_lib():
     L.wfv_batch_set_stream.argtypes = [vp, vp]
     L.wfv_batch_set_rows_per_strip.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.wfv_batch_cells.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    L.wfv_batch_get_rows_per_strip.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.wfv_batch_set_schedule.argtypes = [vp, ctypes.c_int]
+    L.wfv_batch_get_schedule.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.wfv_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Tuning)]
     L._batch_ready = True
     return L
 
@@ -257,6 +325,49 @@ class Batch:
 
     def set_rows_per_strip(self, i, rows):
         self._check(_batch_lib().wfv_batch_set_rows_per_strip(self._h, i, rows))
+
+    def rows_per_strip(self, i):
+        r = ctypes.c_int()
+        self._check(_batch_lib().wfv_batch_get_rows_per_strip(self._h, i, ctypes.byref(r)))
+        return r.value
+
+    def set_schedule(self, persistent):
+        self._check(_batch_lib().wfv_batch_set_schedule(self._h, int(persistent)))
+
+    def schedule(self):
+        p = ctypes.c_int()
+        self._check(_batch_lib().wfv_batch_get_schedule(self._h, ctypes.byref(p)))
+        return p.value
+
+    def _key(self):
+        o = self.opt
+        classes = {}
+        for ny, nx in self.shapes:
+            classes[f"{nx}x{ny}"] = classes.get(f"{nx}x{ny}", 0) + 1
+        sig = ",".join(f"{k}*{v}" for k, v in sorted(classes.items()))
+        return f"batch:{version()}:p{o.precision}r{o.reconstruction}s{o.scheme}f{o.riemann_flux}b{o.pp_limiter}:{sig}"
+
+    def autotune(self, trials=2, cache=True):
+        """Choose rows per strip per grid-size class and the launch schedule by timing the
+        one-launch stage on the current states (left unchanged).  With cache, an earlier
+        choice for the same options and class mix is reused.  Returns {"rows_per_strip":
+        {"NXxNY": rows}, "persistent": 0|1, "stage_ms"}."""
+        key = self._key()
+        hit = _cache_load().get(key) if cache else None
+        if hit:
+            for i, (ny, nx) in enumerate(self.shapes):
+                self.set_rows_per_strip(i, hit["rows_per_strip"][f"{nx}x{ny}"])
+            self.set_schedule(hit["persistent"])
+            return hit
+        t = Tuning()
+        self._check(_batch_lib().wfv_batch_autotune(self._h, trials, ctypes.byref(t)))
+        rows = {}
+        for i, (ny, nx) in enumerate(self.shapes):
+            rows[f"{nx}x{ny}"] = self.rows_per_strip(i)
+        res = {"rows_per_strip": rows, "persistent": t.persistent, "stage_ms": t.stage_ms}
+        if cache:
+            _cache_store(key, res)
+        return res
 
     def cells(self):
         c = ctypes.c_longlong()
