@@ -12,8 +12,10 @@ Workloads
                       limiter, RK3, CFL 0.4, one persistent launch per stage over all
                       instances.  The same batch in all-FP64 gives the baseline throughput
                       and the reference state for the error check (relative L1 per field).
+  tanh_interface      128x128 periodic smooth interface, WENO-JS + LLF, to t = 1, --precision
   liquid_column       1024x512 single case (walls, WENO-Z + HLLC + limiter), --precision
-  perturbed_column    4096x2048 single case, --precision
+  perturbed_column    4096x2048 single case, --precision [--compare]
+  multi_column        8192x8192, 64 columns and a ratio-3 pressure step, --precision [--compare]
 """
 import argparse
 import json
@@ -42,6 +44,7 @@ def parse():
     ap.add_argument("--ty", type=int, default=0, help="rows per wavefront strip (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="batch256: skip the FP64 baseline run")
+    ap.add_argument("--compare", action="store_true", help="single workloads: also run all-FP64 and report errors")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--plain-steps", type=int, default=32,
                     help="batch256: timed steps per instance of the plain FP64 per-instance baseline")
@@ -259,36 +262,74 @@ def run_batch(a, torch, wenofv, cases):
 
 
 def run_single(a, torch, wenofv, cases):
-    if a.workload == "liquid_column":
-        wl = cases.liquid_column(precision=a.precision)
-    elif a.workload == "perturbed_column":
-        wl = cases.perturbed_column(precision=a.precision)
-    else:
+    """One case on the single-domain solver.  Fixed-step workloads are timed over their
+    steps; tanh_interface runs the reference's time loop to its final time.  --compare also
+    runs the case in all-FP64 and reports the field errors of the timed run against it."""
+    makers = {"tanh_interface": lambda: cases.tanh_interface(),
+              "liquid_column": lambda: cases.liquid_column(precision=a.precision),
+              "perturbed_column": lambda: cases.perturbed_column(precision=a.precision),
+              "multi_column": lambda: cases.multi_column(precision=a.precision)}
+    if a.workload not in makers:
         sys.exit(f"unknown workload {a.workload}")
-    steps = a.steps or 200
-    sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty))
-    stream = torch.cuda.Stream()
-    sol.set_stream(stream.cuda_stream)
-    sol.set_state(wl.U0)
-    tiles = sol.autotune(trials=2, cache=False) if not a.ty else {"rows_per_strip": a.ty}
-    if a.warmup:
-        sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
-    sol.set_state(wl.U0)
-    ms, (n, t, _) = timed(torch, sol, stream, steps)
-    assert n == steps, (n, steps)
-    cells = wl.nx * wl.ny
-    U = sol.get_state()
+    wl = makers[a.workload]()
+    if a.workload == "tanh_interface":
+        wl.opts["precision"] = a.precision
     prec = wl.opts.get("precision", 0)
+    steps = a.steps or wl.steps or 0
+    stream = torch.cuda.Stream()
+
+    def run(precision):
+        sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty, precision=precision))
+        sol.set_stream(stream.cuda_stream)
+        sol.set_state(wl.U0)
+        tiles = sol.autotune(trials=2, cache=False) if not a.ty else {"rows_per_strip": a.ty}
+        if a.warmup:
+            sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
+        sol.set_state(wl.U0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        n, t, _ = sol.advance(fixed_steps=steps) if steps else sol.advance(t_end=wl.t_end)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if steps:
+            assert n == steps, (n, steps)
+        return e0.elapsed_time(e1), n, t, sol.get_state(), sol.flags(), tiles
+
+    ms, n, t, U, flags, tiles = run(prec)
+    cells = wl.nx * wl.ny
+    bcname = "walls" if wl.bc == 1 else "periodic"
+    o = wl.opts
+    desc = (f"{wl.name} {wl.nx}x{wl.ny} {bcname} {'WENO-Z' if o['scheme'] else 'WENO-JS'} "
+            f"{'HLLC' if o['flux'] else 'LLF'}{' BP' if o['bp'] else ''} RK3 CFL{o['cfl']}, "
+            + (f"{steps} steps" if steps else f"to t={wl.t_end}"))
     out = {
-        "metric": METRIC, "value": cells * 3 * steps / (ms * 1e-3), "unit": "cell-stage-updates/s", "n_gpus": 1,
-        "steps": steps, "warmup": a.warmup, "ms_per_step": ms / steps, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": cells * 3 * n / (ms * 1e-3), "unit": "cell-stage-updates/s", "n_gpus": 1,
+        "steps": int(n), "warmup": a.warmup, "ms_per_step": ms / n, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32+f64",
         "data": "synthetic (seeded initial condition, no dataset)",
-        "config": {"workload": f"{wl.name} {wl.nx}x{wl.ny} walls WENO-Z HLLC BP RK3 CFL0.4",
-                   "precision": "fp64" if prec == 0 else "mixed", "cells": cells, "final_time": t,
-                   "flags": sol.flags(), "finite": bool(np.isfinite(U).all()), "tiles": tiles},
-        "roofline": roofline(cells, ms / (3 * steps)),
+        "config": {"workload": desc, "precision": "fp64" if prec == 0 else "mixed", "cells": cells,
+                   "final_time": t, "flags": flags, "finite": bool(np.isfinite(U).all()), "tiles": tiles},
+        "roofline": roofline(cells, ms / (3 * n)),
     }
+    z1 = derived(U, **wl.params)[0]
+    out["config"]["z1_range"] = [float(z1.min()), float(z1.max())]
+    m0 = wl.U0[:2].sum(axis=(1, 2))
+    out["config"]["max_phase_mass_drift"] = float(np.abs(U[:2].sum(axis=(1, 2)) / m0 - 1).max())
+    if a.workload == "tanh_interface" and abs(t - 1.0) < 1e-9:
+        # after t = 1 the exact solution is the initial state shifted by (1, 0.5): by half a
+        # period in y
+        z1x = derived(np.roll(wl.U0, wl.ny // 2, axis=1), **wl.params)[0]
+        out["config"]["z1_l1_vs_exact"] = float(np.abs(z1 - z1x).mean())
+    if a.compare and prec == 1:
+        ms64, n64, t64, U64, f64, _ = run(0)
+        errs, linf, _ = field_errors(U, U64, wl.params)
+        gate = ("z1", "rho", "u", "v")
+        out["fp64_gpu"] = {"value": cells * 3 * n64 / (ms64 * 1e-3), "ms_per_step": ms64 / n64, "flags": f64}
+        out["speedup_vs_fp64_gpu"] = out["value"] / out["fp64_gpu"]["value"]
+        out["accuracy"] = {"rel_l1_per_field": errs, "rel_linf": linf, "max_rel_l1": max(errs[k] for k in gate),
+                           "gate_fields": list(gate), "tolerance": 1e-5,
+                           "within_tolerance": max(errs[k] for k in gate) <= 1e-5}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_oracle(wl.nx, wl.ny, wl.domain, wl.bc, wl.params, wl.opts, wl.U0,
                                                   a.cpu_seconds, wl.name)
@@ -298,8 +339,8 @@ def run_single(a, torch, wenofv, cases):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); multi-GPU sharding of this path "
-              "is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU path",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import cases

@@ -180,16 +180,43 @@ def multi_column(nx=8192, ny=8192, ncol=64, seed=5, precision=1):
             cols.append(np.array([c[0], c[1], r]))
     cols = np.array(cols)
     ps, us = shock_state(1.0, 3.0, prm["rho20"], prm["sound2"])
+    x_shock = 0.1
 
-    def pt(X, Y):
+    def pt(X, Y, subset=cols):
         inside = np.zeros(X.shape, dtype=bool)
-        for cx, cy, r in cols:
+        for cx, cy, r in subset:
             inside |= (X - cx) ** 2 + (Y - cy) ** 2 < r * r
         z1 = np.where(inside, 1 - 1e-6, 1e-6)
-        post = X < 0.1
+        post = X < x_shock
         return conservative(z1, np.where(post, ps, 1.0), np.where(post, us, 0.0), 0.0 * X, **prm)
 
     dom = (0.0, 1.0, 0.0, 1.0)
-    U0 = cell_average(nx, ny, dom, pt)
+    # The state is uniform away from the columns and the step: fill the two uniform states,
+    # then average only the cell blocks that the columns or the step touch (the same
+    # quadrature as cell_average on those cells).
+    dx, dy = 1.0 / nx, 1.0 / ny
+    xc = (np.arange(nx) + 0.5) * dx
+    U0 = np.empty((4, ny, nx))
+    U0[:] = conservative(1e-6, 1.0, 0.0, 0.0, **prm)[:, None, None]
+    U0[:, :, xc + 0.5 * dx <= x_shock] = conservative(1e-6, ps, us, 0.0, **prm)[:, None, None]
+
+    def average_block(i0, i1, j0, j1):
+        i0, j0, i1, j1 = max(i0, 0), max(j0, 0), min(i1, nx), min(j1, ny)
+        xs = (np.arange(i0, i1) + 0.5) * dx
+        ys = (np.arange(j0, j1) + 0.5) * dy
+        near = [q for q in cols if q[0] + q[2] >= i0 * dx and q[0] - q[2] <= i1 * dx
+                and q[1] + q[2] >= j0 * dy and q[1] - q[2] <= j1 * dy]
+        acc = np.zeros((4, j1 - j0, i1 - i0))
+        for a in range(4):
+            for b in range(4):
+                X, Y = np.meshgrid(xs + 0.5 * dx * GL_PTS[a], ys + 0.5 * dy * GL_PTS[b])
+                acc += pt(X, Y, near) * (0.5 * GL_WTS[a] * 0.5 * GL_WTS[b])
+        U0[:, j0:j1, i0:i1] = acc
+
+    ishock = int(x_shock / dx)
+    for j in range(0, ny, 1024):
+        average_block(ishock - 1, ishock + 2, j, j + 1024)
+    for cx, cy, r in cols:
+        average_block(int((cx - r) / dx) - 1, int((cx + r) / dx) + 2, int((cy - r) / dy) - 1, int((cy + r) / dy) + 2)
     return Workload("multi_column", nx, ny, dom, prm, U0, 1,
                     dict(scheme=1, recon=0, flux=1, bp=1, rk=2, cfl=0.4, precision=precision), steps=200)
