@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="batch256: skip the FP64 baseline run")
     ap.add_argument("--compare", action="store_true", help="single workloads: also run all-FP64 and report errors")
+    ap.add_argument("--tune-cache", action="store_true",
+                    help="reuse earlier autotuner choices (WENOFV_TUNING_CACHE) instead of timing the candidates again")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--plain-steps", type=int, default=32,
                     help="batch256: timed steps per instance of the plain FP64 per-instance baseline")
@@ -177,7 +179,7 @@ def run_batch(a, torch, wenofv, cases):
         for k, w in enumerate(wls):
             b.set_state(k, w.U0)
         if not a.ty:  # rows per strip per grid class and the schedule from the autotuner
-            tiles[prec] = b.autotune(trials=2, cache=False)
+            tiles[prec] = b.autotune(trials=2, cache=a.tune_cache)
         if a.warmup:
             b.advance(fixed_steps=max(a.warmup, 16))  # also captures the step graph
         for k, w in enumerate(wls):
@@ -282,7 +284,7 @@ def run_single(a, torch, wenofv, cases):
         sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty, precision=precision))
         sol.set_stream(stream.cuda_stream)
         sol.set_state(wl.U0)
-        tiles = sol.autotune(trials=2, cache=False) if not a.ty else {"rows_per_strip": a.ty}
+        tiles = sol.autotune(trials=2, cache=a.tune_cache) if not a.ty else {"rows_per_strip": a.ty}
         if a.warmup:
             sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
         sol.set_state(wl.U0)

@@ -143,7 +143,7 @@ int wfv_batch_cells(const wfv_batch *b, long long *cells);  /* total interior ce
 /* ---------------------------------------------------------------------------------------
    Launch-shape autotuner.  No reference counterpart (the reference's CPU loops have no
    tile shape).  Times `trials` launches of the first RK stage on the current state (which
-   it leaves unchanged) for each candidate rows-per-strip (8..128) and, for a batch, each
+   it leaves unchanged) for each candidate rows-per-strip (1..128) and, for a batch, each
    schedule, and keeps the fastest; a batch is tuned per grid-size class (instances of equal
    nx x ny) by coordinate descent.  Results are also settable directly (a cache of earlier
    tunings): wfv_set_rows_per_strip, wfv_batch_set_rows_per_strip, wfv_batch_set_schedule. */

@@ -1,5 +1,7 @@
-"""The command-line driver (config/settings.cfg, the reference's CLI behaviour) on the GPU:
-its accuracy output reproduces the norms the reference itself wrote for the SMOOTH case."""
+"""The command-line drivers on the GPU: this build's `wenofv_run` and the reference's own,
+unmodified main program built against the CWENOFV facade (oracle/_ref/refmain_gpu, from
+oracle/build_ref.sh).  Both read config/settings.cfg and their accuracy output reproduces
+the norms the reference itself wrote for the SMOOTH case."""
 import glob
 import os
 import subprocess
@@ -12,6 +14,8 @@ from known_answers import KNOWN
 
 pytestmark = pytest.mark.gpu
 CLI = os.path.join(PKG, "wenofv_run")
+REFMAIN = os.path.join(os.path.dirname(PKG), "oracle", "_ref", "refmain_gpu")
+MPI_LIB = "/opt/conda/lib/libmpi.so.12"
 
 SETTINGS = """# SMOOTH = 0
 TESTCASE = 0
@@ -27,18 +31,29 @@ RECONSTRUCTIONTYPE = {recon}
 """
 
 
-@pytest.mark.parametrize("row", KNOWN, ids=lambda k: f"{k[0]}-s{k[1]}r{k[2]}f{k[3]}b{k[4]}")
-def test_cli_reproduces_reference_accuracy(row, tmp_path):
+def run_case(exe, row, tmp_path):
     n, scheme, recon, flux, bp, cfl, linf, l1, l2 = row
     os.makedirs(tmp_path / "config")
     (tmp_path / "config" / "settings.cfg").write_text(
         SETTINGS.format(n=n, cfl=cfl, scheme=scheme, bp=bp, flux=flux, recon=recon))
-    print(f"cli smooth {n} s{scheme} r{recon} f{flux} b{bp}", flush=True)
-    res = subprocess.run([CLI], cwd=tmp_path, capture_output=True, text=True, timeout=100)
+    print(f"{os.path.basename(exe)} smooth {n} s{scheme} r{recon} f{flux} b{bp}", flush=True)
+    res = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=100)
     assert res.returncode == 0, res.stderr
     (csv,) = glob.glob(str(tmp_path / "output" / "*" / "accuracy_final.csv"))
     vals = [float(x) for x in open(csv).read().splitlines()[1].split(",")[1:]]
     np.testing.assert_allclose(vals, (linf, l1, l2), rtol=1e-12)
+
+
+@pytest.mark.parametrize("row", KNOWN, ids=lambda k: f"{k[0]}-s{k[1]}r{k[2]}f{k[3]}b{k[4]}")
+def test_cli_reproduces_reference_accuracy(row, tmp_path):
+    run_case(CLI, row, tmp_path)
+
+
+@pytest.mark.skipif(not (os.path.exists(REFMAIN) and os.path.exists(MPI_LIB)),
+                    reason="the reference's main program was not built here (oracle/build_ref.sh) or MPI is absent")
+@pytest.mark.parametrize("row", KNOWN[:3], ids=lambda k: f"{k[0]}-s{k[1]}r{k[2]}f{k[3]}b{k[4]}")
+def test_reference_main_program_drives_the_library(row, tmp_path):
+    run_case(REFMAIN, row, tmp_path)
 
 
 def test_cli_reports_missing_key(tmp_path):

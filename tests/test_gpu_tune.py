@@ -40,7 +40,7 @@ def test_solver_autotune_keeps_results(precision, tmp_path, monkeypatch):
     assert again.rows_per_strip() == res["rows_per_strip"]
 
 
-@pytest.mark.parametrize("rows", [5, 8, 33, 200])
+@pytest.mark.parametrize("rows", [1, 2, 5, 8, 33, 200])
 def test_solver_rows_per_strip_is_invisible(rows):
     wl = cases.batch_instance(dict(cases.batch_specs(1, seed=6)[0], nx=90, ny=77), precision=0)
     a = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs())

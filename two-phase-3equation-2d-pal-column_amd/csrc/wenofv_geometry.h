@@ -35,27 +35,29 @@ inline void grid_widths(int nx, int ny, double xL, double xR, double yL, double 
 }
 
 // Rows per wavefront strip: long strips amortise the two recomputed halo rows, short ones
-// give more wavefronts; aim for >= 8 wavefronts per CU of a 256-CU device.
+// give more wavefronts; aim for >= 8 wavefronts per CU of a 256-CU device.  Small grids
+// cannot fill the device and take the shortest strips: their stage time is one strip's
+// latency.
 inline int default_rows_per_strip(int nx, int ny) {
     const int sx = (nx + 57) / 58;
-    const int cand[] = {64, 48, 32, 24, 16, 12, 8};
+    const int cand[] = {64, 48, 32, 24, 16, 12, 8, 4, 2};
     for (int ty : cand) {
         const long waves = (long)sx * ((ny + ty - 1) / ty);
         if (waves >= 2048) return ty;
     }
-    return 8;
+    return 1;
 }
 
 // Rows per strip of every instance of a batch: the longest strips that still give the
 // launch at least two rounds of 2048 wavefronts (8 per CU of a 256-CU device).
 inline int default_batch_rows_per_strip(const int *nx, const int *ny, int n) {
-    const int cand[] = {64, 48, 32, 24, 16, 12, 8};
+    const int cand[] = {64, 48, 32, 24, 16, 12, 8, 4, 2};
     for (int ty : cand) {
         long waves = 0;
         for (int i = 0; i < n; ++i) waves += (long)((nx[i] + 57) / 58) * ((ny[i] + ty - 1) / ty);
         if (waves >= 4096) return ty;
     }
-    return 8;
+    return 1;
 }
 
 }  // namespace wfv

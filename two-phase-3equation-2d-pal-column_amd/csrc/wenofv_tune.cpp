@@ -25,7 +25,7 @@ using namespace wfv;
 
 namespace {
 
-constexpr int CANDIDATES[] = {8, 12, 16, 24, 32, 48, 64, 96, 128};
+constexpr int CANDIDATES[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
 constexpr int REPEATS = 3;
 
 // candidate rows per strip for a grid of ny rows
