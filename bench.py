@@ -116,14 +116,13 @@ def timed(torch, obj, stream, steps):
     return e0.elapsed_time(e1), res
 
 
-def profiled_traffic(name):
-    """HBM bytes per stage launch from a committed rocprofv3 counter summary (FETCH_SIZE and
-    WRITE_SIZE, scaled by a device copy's readings), or None."""
+def profiled(name, key):
+    """A machine-readable figure of a committed rocprofv3 summary under profiles/, or None."""
     path = os.path.join(REPO, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
@@ -198,9 +197,16 @@ def run_batch(a, torch, wenofv, cases):
                                f"mixed WENO-Z HLLC BP RK3 CFL0.4, {steps} steps each, one launch per stage",
                    "instances": len(wls), "cells": cells, "flags": results["mixed"]["flags"],
                    "tiles": tiles.get(runs[0][1], {"rows_per_strip": a.ty})},
-        "roofline": roofline(cells, results["mixed"]["ms"] / (3 * steps),
-                             profiled_traffic(f"batch_stage_{'fp64' if runs[0][1] == 0 else 'mixed'}_batch256.md")),
     }
+    prof = f"batch_stage_{'fp64' if runs[0][1] == 0 else 'mixed'}_batch256.md"
+    out["roofline"] = roofline(cells, results["mixed"]["ms"] / (3 * steps), profiled(prof, "traffic_bytes_per_launch"))
+    # the bound that applies: VALU issue (rocprofv3 counters of the same kernel and workload)
+    out["compute"] = {"bound": "valu", "issue_frac": profiled(prof, "valu_issue_frac"),
+                      "lane_instructions_per_cell_stage": profiled(prof, "lane_instructions_per_cell_stage"),
+                      "profile_stage_ms": (profiled(prof, "stage_kernel_avg_ns") or 0) / 1e6 or None,
+                      "bench_stage_ms": results["mixed"]["ms"] / (3 * steps), "source": f"profiles/{prof}",
+                      "note": "issue_frac = SQ_INSTS_VALU per SIMD cycle / 0.25 (a wave64 VALU instruction "
+                              "occupies a SIMD for 4 cycles)"}
     if not a.no_fp64:
         plain_rate, plain_ms = plain_fp64_per_instance(torch, wenofv, wls, opts, stream, a.plain_steps)
         out["fp64_plain_gpu"] = {

@@ -203,27 +203,24 @@ WFV_DEV void riemann_hllc(const Phys &P, const double dl[4], const double dr[4],
     // with pref = p_L the shifted pressures already form the jump pre_R - pre_L
     const T S_star = dv((S_L - vn_L) * rho_L * vn_L - (S_R - vn_R) * rho_R * vn_R + pre_R - pre_L,
                         (S_L - vn_L) * rho_L - (S_R - vn_R) * rho_R);
-    if (S_star >= T(0)) {
-        if (S_L >= T(0)) {
+    // The reference's four cases (S* >= 0: F_L or the left star flux; S* < 0: F_R or the
+    // right star flux) as one star-flux evaluation on the upwind side's values: the taken
+    // case performs the same operations, and a wavefront whose lanes take different cases
+    // evaluates one star state instead of two.
+    const bool left = S_star >= T(0);
+    const bool outside = left ? (S_L >= T(0)) : (S_R <= T(0));
+    const T S_K = left ? S_L : S_R, vn_K = left ? vn_L : vn_R, rho_K = left ? rho_L : rho_R;
+    const T u_K = left ? u_L : u_R, v_K = left ? v_L : v_R;
+    T U_K[4], F_K[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) F[r] = FL[r];
-        } else {
-            const T k = dv(S_L - vn_L, S_L - S_star);
-            T Us[4] = {UL[0], UL[1], rho_L * (u_L + (S_star - vn_L) * nx), rho_L * (v_L + (S_star - vn_L) * ny)};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) F[r] = S_L * (k * Us[r] - UL[r]) + FL[r];
-        }
-    } else {
-        if (S_R <= T(0)) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) F[r] = FR[r];
-        } else {
-            const T k = dv(S_R - vn_R, S_R - S_star);
-            T Us[4] = {UR[0], UR[1], rho_R * (u_R + (S_star - vn_R) * nx), rho_R * (v_R + (S_star - vn_R) * ny)};
-#pragma unroll
-            for (int r = 0; r < 4; ++r) F[r] = S_R * (k * Us[r] - UR[r]) + FR[r];
-        }
+    for (int r = 0; r < 4; ++r) {
+        U_K[r] = left ? UL[r] : UR[r];
+        F_K[r] = left ? FL[r] : FR[r];
     }
+    const T k = dv(S_K - vn_K, S_K - S_star);
+    const T Us[4] = {U_K[0], U_K[1], rho_K * (u_K + (S_star - vn_K) * nx), rho_K * (v_K + (S_star - vn_K) * ny)};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) F[r] = outside ? F_K[r] : S_K * (k * Us[r] - U_K[r]) + F_K[r];
 }
 
 // getRiemannFlux dispatch (include/Equations.hpp:329-360), resolved at compile time
@@ -333,21 +330,27 @@ struct Weno5 {
             const T mx = wMax(wMax(wMax(a, b), wMax(e, d)), c);
             const T mn = wMin(wMin(wMin(a, b), wMin(e, d)), c);
             const T den = mx - mn + T(1e-20);
-            T a_, b_, c_, d_, e_;
             if constexpr (sizeof(T) == 8) {
-                a_ = (a - mn) / den; b_ = (b - mn) / den; c_ = (c - mn) / den;
-                d_ = (d - mn) / den; e_ = (e - mn) / den;
+                const T a_ = (a - mn) / den, b_ = (b - mn) / den, c_ = (c - mn) / den;
+                const T d_ = (d - mn) / den, e_ = (e - mn) / den;
+                beta3(a_, b_, c_, d_, e_, b0, b1, b2);
+                const T eps = T(1.0e-10);
+                const T tau = fabs(b0 - b2);
+                k0 = dv(tau, b0 + eps);
+                k1 = dv(tau, b1 + eps);
+                k2 = dv(tau, b2 + eps);
             } else {
-                const T id = __builtin_amdgcn_rcpf(den);
-                a_ = (a - mn) * id; b_ = (b - mn) * id; c_ = (c - mn) * id;
-                d_ = (d - mn) * id; e_ = (e - mn) * id;
+                // The indicators are shift-invariant quadratic forms, so those of the
+                // normalised stencil are beta/den^2 and tau/(beta_k + eps) on it equals
+                // tau_raw/(beta_k,raw + eps den^2) on the raw one.  The floor keeps a
+                // constant stencil at k = 0 where eps den^2 underflows.
+                beta3(a, b, c, d, e, b0, b1, b2);
+                const T e2 = wMax(T(1.0e-10) * den * den, T(1.17549435e-38f));
+                const T tau = fabs(b0 - b2);
+                k0 = tau * __builtin_amdgcn_rcpf(b0 + e2);
+                k1 = tau * __builtin_amdgcn_rcpf(b1 + e2);
+                k2 = tau * __builtin_amdgcn_rcpf(b2 + e2);
             }
-            beta3(a_, b_, c_, d_, e_, b0, b1, b2);
-            const T eps = T(1.0e-10);
-            const T tau = fabs(b0 - b2);
-            k0 = dv(tau, b0 + eps);
-            k1 = dv(tau, b1 + eps);
-            k2 = dv(tau, b2 + eps);
         }
     }
 
