@@ -44,6 +44,11 @@ def _load():
     P = ctypes.POINTER(Params)
     lib.orc_weno.restype = ctypes.c_double
     lib.orc_weno.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+    lib.orc_weno7.restype = ctypes.c_double
+    lib.orc_weno7.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double]
+    for name in ("orc_weno_exp", "orc_weno_exp_thinc"):
+        getattr(lib, name).restype = ctypes.c_double
+        getattr(lib, name).argtypes = [_dp, ctypes.c_int, ctypes.c_double]
     for name in ("orc_volume_frac", "orc_pressure", "orc_sound_speed"):
         getattr(lib, name).restype = ctypes.c_double
         getattr(lib, name).argtypes = [P, _dp]
@@ -180,3 +185,16 @@ class Oracle:
 
 def weno(scheme, s, gp):
     return lib().orc_weno(scheme, _ptr(_vec(s, 5)), gp)
+
+
+def weno7(scheme, s, gp, dx):
+    """useWENO on a 7-point stencil (SCHEME 0, 1, 2)."""
+    return lib().orc_weno7(scheme, _ptr(_vec(s, 7)), gp, dx)
+
+
+def weno_exp(s, gp, dx):
+    return lib().orc_weno_exp(_ptr(_vec(s, 7)), gp, dx)
+
+
+def weno_exp_thinc(s, gp, dx):
+    return lib().orc_weno_exp_thinc(_ptr(_vec(s, 7)), gp, dx)

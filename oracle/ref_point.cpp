@@ -43,6 +43,13 @@ int ref_set_case(int testcase, double *out5) {
 
 double ref_weno_js(const double *s, int gp) { return WENO5threconstruction(s[0], s[1], s[2], s[3], s[4], gp); }
 double ref_weno_z(const double *s, int gp) { return WENO5Zthreconstruction(s[0], s[1], s[2], s[3], s[4], gp); }
+// 7-point kernels (s = {u_mmm, ..., u_ppp}); dx is the reference's m_deltaX
+double ref_weno_exp(const double *s, int gp, double dx) {
+    return wenoExpthreconstruction(s[0], s[1], s[2], s[3], s[4], s[5], s[6], gp, dx);
+}
+double ref_weno_exp_thinc(const double *s, int gp, double dx) {
+    return wenoExpWithThincReconstruction(s[0], s[1], s[2], s[3], s[4], s[5], s[6], gp, dx);
+}
 
 double ref_volume_frac(const double *u) { return g_eq.getVolumeFrac(vec4(u)); }
 double ref_pressure(const double *u) { return g_eq.getPressure(vec4(u)); }

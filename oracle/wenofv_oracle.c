@@ -124,6 +124,126 @@ static double weno_z(double a, double b, double c, double d, double e, int gp) {
     return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
+/* ------------------------------------------------ 7-point kernels (WENO-EXP, THINC) */
+
+/* fitted linear weights of WENO-EXP, cubic in l2h2 (include/Utility.hpp:347-380, 545-578) */
+static void wexp_d(int gp, double l, double *d0, double *d1, double *d2) {
+    switch (gp) {
+    case 0:
+        *d0 = 0.3 - 0.057142858 * l + 0.0071428571 * pow(l, 2) - 0.00077200577 * pow(l, 3);
+        *d1 = 0.6 + 0.078571429 * l - 0.0101190476 * pow(l, 2) + 0.00112012987 * pow(l, 3);
+        *d2 = 0.1 - 0.021428571 * l + 0.0029761905 * pow(l, 2) - 0.00034812410 * pow(l, 3);
+        break;
+    case 1:
+        *d0 = 0.16108042 - 0.035187607 * l + 0.0049466098 * pow(l, 2) - 0.00058211525 * pow(l, 3);
+        *d1 = 0.58636364 + 0.087692641 * l - 0.0120401515 * pow(l, 2) + 0.00139436871 * pow(l, 3);
+        *d2 = 0.25255594 - 0.052505034 * l + 0.0070935417 * pow(l, 2) - 0.00081225346 * pow(l, 3);
+        break;
+    case 2:
+        *d0 = 0.25255594 - 0.052505034 * l + 0.0070935417 * pow(l, 2) - 0.00081225346 * pow(l, 3);
+        *d1 = 0.58636364 + 0.087692641 * l - 0.0120401515 * pow(l, 2) + 0.00139436871 * pow(l, 3);
+        *d2 = 0.16108042 - 0.035187607 * l + 0.0049466098 * pow(l, 2) - 0.00058211525 * pow(l, 3);
+        break;
+    default:
+        *d0 = 0.1 - 0.021428571 * l + 0.0029761905 * pow(l, 2) - 0.00034812410 * pow(l, 3);
+        *d1 = 0.6 + 0.078571429 * l - 0.0101190476 * pow(l, 2) + 0.00112012987 * pow(l, 3);
+        *d2 = 0.3 - 0.057142858 * l + 0.0071428571 * pow(l, 2) - 0.00077200577 * pow(l, 3);
+        break;
+    }
+}
+
+/* fifth and third differences of the 6 cells upwind-biased toward node gp
+   (include/Utility.hpp:297-307, 486-496); s = {u_mmm, ..., u_ppp} */
+static void wexp_dif(const double s[7], int gp, double *dif5, double *dif3) {
+    const double *t = gp >= 2 ? s + 1 : s;
+    *dif5 = (-t[0] + 5.0 * t[1] - 10.0 * t[2] + 10.0 * t[3] - 5.0 * t[4] + t[5]);
+    *dif3 = (t[0] - 11.0 * t[1] + 28.0 * t[2] - 28.0 * t[3] + 11.0 * t[4] - t[5]) / 6.0;
+}
+
+/* min-max normalisation of the inner 5 cells (include/Utility.hpp:393-401) */
+static void wexp_norm(const double s[7], double n[5]) {
+    const double mx = stdMax(stdMax(stdMax(s[1], s[2]), stdMax(s[3], s[4])), s[5]);
+    const double mn = stdMin(stdMin(stdMin(s[1], s[2]), stdMin(s[3], s[4])), s[5]);
+    for (int k = 0; k < 5; ++k) n[k] = (s[1 + k] - mn) / (mx - mn + 1e-20);
+}
+
+/* wenoExpthreconstruction (include/Utility.hpp:291-472): WENO-EXP with l2h2 from the
+   fifth/third difference ratio (clamped to 1 in magnitude, 0 where the third difference is
+   below dx^4) and the exponential-type indicators on the normalised stencil */
+double orc_weno_exp(const double s[7], int gp, double dx) {
+    double dif5, dif3;
+    wexp_dif(s, gp, &dif5, &dif3);
+    double l2h2 = 0.0;
+    if (!(fabs(dif3) < pow(dx, 4))) {
+        l2h2 = dif5 / dif3;
+        if (fabs(l2h2) > 1) l2h2 = 1;
+    }
+    double h0, h1, h2, d0, d1, d2, n[5];
+    weno_h(gp, s[1], s[2], s[3], s[4], s[5], &h0, &h1, &h2);
+    wexp_d(gp, l2h2, &d0, &d1, &d2);
+    wexp_norm(s, n);
+    const double theta = 1;
+    const double beta0 = theta * 1.0 / 2.0 * fabs(n[0] - 4 * n[1] + 3 * n[2]) + fabs(n[0] - 2 * n[1] + n[2]);
+    const double beta1 = theta * 1.0 / 2.0 * fabs(n[1] - n[3]) + fabs(n[1] - 2 * n[2] + n[3]);
+    const double beta2 = theta * 1.0 / 2.0 * fabs(3 * n[2] - 4 * n[3] + n[4]) + fabs(n[2] - 2 * n[3] + n[4]);
+    const double tau = n[0] - 4.0 * n[1] + 6.0 * n[2] - 4.0 * n[3] + n[4];
+    const double epsilon = 1e-16;
+    double w0 = d0 * (1.0 + 2 * pow(tau, 2.0) / (pow(beta0, 2.0) + epsilon));
+    double w1 = d1 * (1.0 + 2 * pow(tau, 2.0) / (pow(beta1, 2.0) + epsilon));
+    double w2 = d2 * (1.0 + 2 * pow(tau, 2.0) / (pow(beta2, 2.0) + epsilon));
+    const double S = w0 + w1 + w2;
+    w0 /= S; w1 /= S; w2 /= S;
+    return w0 * h0 + w1 * h1 + w2 * h2;
+}
+
+/* Gauss-Lobatto-4 points on [-1,1] (third_party/scTools/GaussQuadratureInRef.hpp:252-262) */
+static const double GLP[4] = {-1, -0.447213595499957939281834733746, 0.447213595499957939281834733746, 1};
+
+/* wenoExpWithThincReconstruction (include/Utility.hpp:473-663): in cells whose value lies
+   strictly inside (1e-6, 1-1e-6) a THINC tanh profile through the 3-cell stencil, elsewhere
+   WENO-Z indicators (eps 1e-16) with the WENO-EXP linear weights (l2h2 not clamped) */
+double orc_weno_exp_thinc(const double s[7], int gp, double dx) {
+    double dif5, dif3;
+    wexp_dif(s, gp, &dif5, &dif3);
+    double l2h2 = 0.0;
+    if (!(fabs(dif3) < pow(dx, 4))) l2h2 = dif5 / dif3;
+    const double um = s[2], uc = s[3], up = s[4];
+    if (!((1e-6 < uc) & (uc < 1 - 1e-6))) {
+        double h0, h1, h2, d0, d1, d2, n[5];
+        weno_h(gp, s[1], s[2], s[3], s[4], s[5], &h0, &h1, &h2);
+        wexp_d(gp, l2h2, &d0, &d1, &d2);
+        wexp_norm(s, n);
+        const double beta0 = 0.25 * pow(n[0] - 4 * n[1] + 3 * n[2], 2) + 13.0 / 12.0 * pow(n[0] - 2 * n[1] + n[2], 2);
+        const double beta1 = 0.25 * pow(n[1] - n[3], 2) + 13.0 / 12.0 * pow(n[1] - 2 * n[2] + n[3], 2);
+        const double beta2 = 0.25 * pow(3 * n[2] - 4 * n[3] + n[4], 2) + 13.0 / 12.0 * pow(n[2] - 2 * n[3] + n[4], 2);
+        const double eps = 1.0e-16;
+        const double tau = fabs(beta0 - beta2);
+        double w0 = d0 * (1 + tau / (beta0 + eps));
+        double w1 = d1 * (1 + tau / (beta1 + eps));
+        double w2 = d2 * (1 + tau / (beta2 + eps));
+        const double S = w0 + w1 + w2;
+        w0 /= S; w1 /= S; w2 /= S;
+        return w0 * h0 + w1 * h1 + w2 * h2;
+    }
+    const double Qmin = stdMin(um, up);
+    const double Qmax = stdMax(um, up) - Qmin;
+    const double theta = (up - uc >= 0) ? 1 : -1; /* sc_math::Sign (MyMath.h:47-50) */
+    const double beta = 1.5;
+    const double C = (uc - Qmin + 1e-20) / (Qmax + 1e-20);
+    const double B = exp(theta * beta * (2.0 * C - 1.0));
+    const double xc = (exp(beta) - B) / (B - exp(-beta));
+    const double xcc = log(xc) / (2.0 * beta);
+    if ((uc - um) * (up - uc) < 0) return uc;
+    return 0.5 * Qmax * (1 + theta * tanh(beta * ((GLP[gp] + 1) / 2 - xcc))) + Qmin;
+}
+
+/* CWENOFV::useWENO (src/WENOFV.cpp:1873-1893) on the 7-point stencil s = {u_mmm, ..., u_ppp};
+   dx is the reference's m_deltaX (WENO-EXP's smoothness threshold) */
+double orc_weno7(int scheme, const double s[7], int gp, double dx) {
+    if (scheme == 2) return orc_weno_exp(s, gp, dx);
+    return scheme == 1 ? weno_z(s[1], s[2], s[3], s[4], s[5], gp) : weno_js(s[1], s[2], s[3], s[4], s[5], gp);
+}
+
 /* CWENOFV::useWENO dispatch (src/WENOFV.cpp:1873-1893); s = {u_mm, u_m, u, u_p, u_pp} */
 double orc_weno(int scheme, const double s[5], int gp) {
     return scheme == 1 ? weno_z(s[0], s[1], s[2], s[3], s[4], gp)
@@ -399,9 +519,13 @@ void orc_geometry(int nx, int ny, double xL, double xR, double yL, double yR, in
 
 /* ---------------------------------------------------------------- stencil sweeps */
 
+/* ghost layers of the padded state: the 7-point stencils (WENO-EXP, primitive3) of the
+   ring cells reach 4 cells out; the 5-point schemes read the inner 5 of the same stencil */
+#define PAD 4
+
 typedef struct {
     const orc_params *p;
-    int nx, ny, PX, PY;   /* padded extents (3 ghost layers) */
+    int nx, ny, PX, PY;   /* padded extents (PAD ghost layers) */
     double *Up;           /* [PY][PX][4] padded cell averages         */
     double *Wp;           /* [PY][PX][4] reconstructed variables (primitive modes) */
     double *Y;            /* [(ny+2)][PX][4 g][4 v]: y-sweep, rows -1..ny */
@@ -410,9 +534,9 @@ typedef struct {
     int err;
 } work;
 
-static inline double *UP(work *w, int i, int j) { return w->Up + ((size_t)(j + 3) * w->PX + (i + 3)) * 4; }
-static inline double *WP(work *w, int i, int j) { return w->Wp + ((size_t)(j + 3) * w->PX + (i + 3)) * 4; }
-static inline double *YP(work *w, int i, int j) { return w->Y + ((size_t)(j + 1) * w->PX + (i + 3)) * 16; }
+static inline double *UP(work *w, int i, int j) { return w->Up + ((size_t)(j + PAD) * w->PX + (i + PAD)) * 4; }
+static inline double *WP(work *w, int i, int j) { return w->Wp + ((size_t)(j + PAD) * w->PX + (i + PAD)) * 4; }
+static inline double *YP(work *w, int i, int j) { return w->Y + ((size_t)(j + 1) * w->PX + (i + PAD)) * 16; }
 static inline double *FP(work *w, int i, int j) { return w->Fc + ((size_t)(j + 1) * (w->nx + 2) + (i + 1)) * 64; }
 
 /* ghost-cell fill: periodic wrap as exchangeGhostCellsValue + setBoundaryAverages2
@@ -421,8 +545,8 @@ static inline double *FP(work *w, int i, int j) { return w->Fc + ((size_t)(j + 1
 static void fill_padded(work *w, const double *U) {
     const int nx = w->nx, ny = w->ny;
     const size_t nc = (size_t)nx * ny;
-    for (int j = -3; j < ny + 3; ++j) {
-        for (int i = -3; i < nx + 3; ++i) {
+    for (int j = -PAD; j < ny + PAD; ++j) {
+        for (int i = -PAD; i < nx + PAD; ++i) {
             int si = i, sj = j;
             double sgnu = 1.0, sgnv = 1.0;
             if (w->p->bc == 1) {
@@ -482,28 +606,37 @@ static void from_primitive(const orc_params *p, const double W[4], double U[4]) 
     U[0] = z1rho1; U[1] = z2rho2; U[2] = rho * W[2]; U[3] = rho * W[3];
 }
 
+/* reconstruction of variable v at node gp from a 7-point stencil: useWENO for RECON 0-3;
+   for PRIMITIVE3 wenoExpWithThincReconstruction on z1 and WENO5Zthreconstruction on
+   p, u, v whatever SCHEME says (src/WENOFV.cpp:1287-1329) */
+static double recon7(const orc_params *p, const double s[7], int gp, int v) {
+    if (p->recon == 4)
+        return v == 0 ? orc_weno_exp_thinc(s, gp, p->dX) : weno_z(s[1], s[2], s[3], s[4], s[5], gp);
+    return orc_weno7(p->scheme, s, gp, p->dX);
+}
+
 /* y-direction sweep of one column x-offset (steps 2-4 of getFlux_*):
    Y[g*4+v] = value at y-node g of column i, row j (src/WENOFV.cpp:670-687, 796-833). */
 static void sweep_y(work *w, int i, int j, double *Yo) {
     const orc_params *p = w->p;
-    double s[5];
+    double s[7];
     if (p->recon == 1) {
-        double L[16], R[16], Uc[5][4], Yc[4][4];
+        double L[16], R[16], Uc[7][4], Yc[4][4];
         orc_eigen_L(p, UP(w, i, j), 0, 1, L);
-        for (int t = 0; t < 5; ++t) matvec(L, UP(w, i, j - 2 + t), Uc[t]);
+        for (int t = 0; t < 7; ++t) matvec(L, UP(w, i, j - 3 + t), Uc[t]);
         for (int g = 0; g < NG; ++g)
             for (int v = 0; v < NV; ++v) {
-                for (int t = 0; t < 5; ++t) s[t] = Uc[t][v];
-                Yc[g][v] = orc_weno(p->scheme, s, g);
+                for (int t = 0; t < 7; ++t) s[t] = Uc[t][v];
+                Yc[g][v] = recon7(p, s, g, v);
             }
         orc_eigen_R(p, UP(w, i, j), 0, 1, R);
         for (int g = 0; g < NG; ++g) matvec(R, Yc[g], Yo + g * 4);
     } else {
         for (int g = 0; g < NG; ++g)
             for (int v = 0; v < NV; ++v) {
-                for (int t = 0; t < 5; ++t)
-                    s[t] = (p->recon == 0) ? UP(w, i, j - 2 + t)[v] : WP(w, i, j - 2 + t)[v];
-                Yo[g * 4 + v] = orc_weno(p->scheme, s, g);
+                for (int t = 0; t < 7; ++t)
+                    s[t] = (p->recon == 0) ? UP(w, i, j - 3 + t)[v] : WP(w, i, j - 3 + t)[v];
+                Yo[g * 4 + v] = recon7(p, s, g, v);
             }
     }
 }
@@ -513,21 +646,21 @@ static void sweep_y(work *w, int i, int j, double *Yo) {
 static void sweep_x(work *w, int i, int j) {
     const orc_params *p = w->p;
     double Ugp[4][4][4]; /* [gx][gy][v] */
-    double s[5];
-    const double *Yw[5];
-    for (int t = 0; t < 5; ++t) Yw[t] = YP(w, i - 2 + t, j);
+    double s[7];
+    const double *Yw[7];
+    for (int t = 0; t < 7; ++t) Yw[t] = YP(w, i - 3 + t, j);
     if (p->recon == 1) {
-        double L[16], R[16], Yc[5][4][4], Uc[4];
+        double L[16], R[16], Yc[7][4][4], Uc[4];
         orc_eigen_L(p, UP(w, i, j), 1, 0, L);
         orc_eigen_R(p, UP(w, i, j), 1, 0, R);
-        for (int t = 0; t < 5; ++t)
+        for (int t = 0; t < 7; ++t)
             for (int g = 0; g < NG; ++g) matvec(L, Yw[t] + g * 4, Yc[t][g]);
         for (int gy = 0; gy < NG; ++gy)
             for (int gx = 0; gx < NG; ++gx) {
                 if (!(gx == 0 || gx == 3 || gy == 0 || gy == 3)) continue;
                 for (int v = 0; v < NV; ++v) {
-                    for (int t = 0; t < 5; ++t) s[t] = Yc[t][gy][v];
-                    Uc[v] = orc_weno(p->scheme, s, gx);
+                    for (int t = 0; t < 7; ++t) s[t] = Yc[t][gy][v];
+                    Uc[v] = recon7(p, s, gx, v);
                 }
                 matvec(R, Uc, Ugp[gx][gy]);
             }
@@ -536,8 +669,8 @@ static void sweep_x(work *w, int i, int j) {
             for (int gx = 0; gx < NG; ++gx) {
                 if (!(gx == 0 || gx == 3 || gy == 0 || gy == 3)) continue;
                 for (int v = 0; v < NV; ++v) {
-                    for (int t = 0; t < 5; ++t) s[t] = Yw[t][gy * 4 + v];
-                    Ugp[gx][gy][v] = orc_weno(p->scheme, s, gx);
+                    for (int t = 0; t < 7; ++t) s[t] = Yw[t][gy * 4 + v];
+                    Ugp[gx][gy][v] = recon7(p, s, gx, v);
                 }
                 if (p->recon >= 2) {
                     double Wn[4];
@@ -576,7 +709,7 @@ static int work_init(work *w, const orc_params *p) {
     memset(w, 0, sizeof *w);
     w->p = p;
     w->nx = p->nx; w->ny = p->ny;
-    w->PX = p->nx + 6; w->PY = p->ny + 6;
+    w->PX = p->nx + 2 * PAD; w->PY = p->ny + 2 * PAD;
     w->Up = (double *)malloc(sizeof(double) * 4 * w->PX * w->PY);
     w->Wp = (double *)malloc(sizeof(double) * 4 * w->PX * w->PY);
     w->Y = (double *)malloc(sizeof(double) * 16 * w->PX * (p->ny + 2));
@@ -596,16 +729,13 @@ static int assemble(work *w, const double *U, double *rhs, double *faces_out, do
     w->err = 0;
     fill_padded(w, U);
     if (p->recon >= 2) {
-        for (int j = -3; j < ny + 3; ++j)
-            for (int i = -3; i < nx + 3; ++i) to_primitive(p, UP(w, i, j), WP(w, i, j));
+        for (int j = -PAD; j < ny + PAD; ++j)
+            for (int i = -PAD; i < nx + PAD; ++i) to_primitive(p, UP(w, i, j), WP(w, i, j));
     }
-    /* y-sweep: columns -3..nx+2 on rows -1..ny (rows -1 and ny only feed the ring cells) */
+    /* y-sweep: every padded column on rows -1..ny (rows -1 and ny only feed the ring cells) */
 #pragma omp parallel for schedule(static)
     for (int j = -1; j <= ny; ++j)
-        for (int i = -3; i < nx + 3; ++i) {
-            if ((j == -1 || j == ny) && (i < -2 || i > nx + 1)) continue;
-            sweep_y(w, i, j, YP(w, i, j));
-        }
+        for (int i = -PAD; i < nx + PAD; ++i) sweep_y(w, i, j, YP(w, i, j));
     /* x-sweep + limiter: interior cells and the one-cell ring without corners */
     int err = 0;
 #pragma omp parallel for schedule(static) reduction(| : err)

@@ -13,6 +13,8 @@
  * Pinning (how each function is checked, tests/test_oracle_*.py):
  *   orc_weno          golden vectors from the compiled reference (tests/golden/point_*.npz),
  *                     polynomial exactness, constant-stencil, JS/Z equal-beta identity
+ *   orc_weno_exp(_thinc)  golden vectors from the reference's 7-point kernels
+ *                     (tests/golden/point_wenoexp.npz)
  *   orc_eos / flux    golden vectors from the reference, p1(z1)=p2(z1) equilibrium,
  *                     Wood sound speed closed form, F(U,U)=F_phys(U) consistency
  *   orc_eigen_L/R     golden vectors, L*R = I
@@ -32,8 +34,9 @@ extern "C" {
 
 typedef struct orc_params {
     int nx, ny;        /* interior cells */
-    int scheme;        /* 0 WENO-JS, 1 WENO-Z                                   */
-    int recon;         /* 0 conservative, 1 characteristic, 2 primitive, 3 primitive2 */
+    int scheme;        /* 0 WENO-JS, 1 WENO-Z, 2 WENO-EXP                       */
+    int recon;         /* 0 conservative, 1 characteristic, 2 primitive, 3 primitive2,
+                          4 primitive3 (THINC on z1) */
     int flux;          /* 0 LLF, 1 HLLC                                          */
     int bp;            /* bound-preserving limiter on partial densities          */
     int bc;            /* 0 periodic, 1 reflective walls                         */
@@ -55,6 +58,10 @@ typedef struct orc_params {
 
 /* point kernels */
 double orc_weno(int scheme, const double s[5], int gp);
+/* 7-point stencils s = {u_mmm, ..., u_ppp}; dx is the reference's m_deltaX */
+double orc_weno7(int scheme, const double s[7], int gp, double dx);       /* useWENO, SCHEME 0-2 */
+double orc_weno_exp(const double s[7], int gp, double dx);               /* WENO-EXP            */
+double orc_weno_exp_thinc(const double s[7], int gp, double dx);         /* z1 of PRIMITIVE3    */
 double orc_volume_frac(const orc_params *p, const double U[4]);
 double orc_pressure(const orc_params *p, const double U[4]);
 double orc_sound_speed(const orc_params *p, const double U[4]);

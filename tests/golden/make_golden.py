@@ -299,6 +299,106 @@ def make_runs():
     print("run fixtures written")
 
 
+def exp_stencils(rng):
+    """7-point stencil classes for WENO-EXP and the THINC kernel: random, smooth (polynomial
+    cell averages, small third differences), steps, near-constant, scaled, and volume
+    fractions across an interface (values inside and at the edges of (1e-6, 1-1e-6))."""
+    out = [rng.standard_normal((150, 7))]
+    x = np.arange(-3, 4, dtype=np.float64)
+    for _ in range(60):
+        c = rng.standard_normal(6) * np.array([1, 1, 0.5, 0.1, 0.01, 0.001])
+        avg = [sum(c[m] * (((xk + 0.5) ** (m + 1) - (xk - 0.5) ** (m + 1)) / (m + 1)) for m in range(6)) for xk in x]
+        out.append(np.array(avg)[None, :])
+    steps = []
+    for k in range(1, 7):
+        for lo, hi in ((0.0, 1.0), (1.0, 0.0), (1e-8, 1 - 1e-8), (1000.0, 1.0)):
+            t = np.full(7, lo)
+            t[k:] = hi
+            steps.append(t)
+    out.append(np.array(steps))
+    out.append(1.0 + 1e-9 * rng.standard_normal((30, 7)))
+    out.append(1e3 * rng.standard_normal((30, 7)))
+    # interface profiles of a volume fraction: tanh fronts of random width and offset
+    xs = np.arange(-3, 4)
+    prof = []
+    for _ in range(150):
+        w, o, sgn = 10 ** rng.uniform(-1, 0.7), rng.uniform(-2, 2), rng.choice([-1, 1])
+        z = 0.5 + 0.5 * np.tanh(sgn * (xs - o) / w)
+        prof.append(np.clip(z, 1e-8, 1 - 1e-8))
+    out.append(np.array(prof))
+    edge = np.tile(np.array([1e-8, 1e-7, 5e-7, 1e-6, 2e-6, 1e-5, 0.5]), (10, 1))
+    edge += 1e-7 * rng.random(edge.shape)
+    out.append(edge)
+    out.append(1 - edge)
+    out.append(np.zeros((1, 7)))
+    out.append(np.full((1, 7), 0.25))
+    return np.ascontiguousarray(np.vstack(out))
+
+
+def make_point_exp(rng):
+    lib = refpoint()
+    for n in ("ref_weno_exp", "ref_weno_exp_thinc"):
+        getattr(lib, n).restype = ctypes.c_double
+        getattr(lib, n).argtypes = [_dp, ctypes.c_int, ctypes.c_double]
+    S = exp_stencils(rng)
+    dxs = np.array([1.0 / 16, 1.0 / 64, 1.0 / 100, 0.5, 2.0 / 256])
+    exp_v = np.array([[[lib.ref_weno_exp(ptr(s), g, dx) for g in range(4)] for s in S] for dx in dxs])
+    thinc = np.array([[[lib.ref_weno_exp_thinc(ptr(s), g, dx) for g in range(4)] for s in S] for dx in dxs])
+    np.savez_compressed(os.path.join(HERE, "point_wenoexp.npz"), stencils=S, dx=dxs, exp=exp_v, thinc=thinc)
+    print("WENO-EXP point fixtures written")
+
+
+def make_rhs_exp(rng):
+    """assembleRHS with SCHEME=2 (WENO-EXP, every reconstruction form) and
+    RECONSTRUCTIONTYPE=4 (primitive3: THINC on z1)."""
+    nx = ny = 12
+    out = {}
+    for kind, params in (("smooth", (1.2, 1.0, 1.0, 1.0, 0.0)), ("sharp", (1000.0, 1.0, 70.0, 70.0, 0.0))):
+        U0 = random_state(rng, nx, ny, params, kind)
+        out[f"{kind}_U"] = U0
+        out[f"{kind}_params"] = np.array(params)
+        combos = [(2, r) for r in (0, 1, 2, 3)] + [(1, 4)]
+        for scheme, recon in combos:
+            for flux in (0, 1):
+                for bp in (0, 1):
+                    if kind == "sharp" and (bp == 0 or recon >= 2):
+                        continue  # negative partial densities without the limiter: NaN in the reference
+                    tag = f"{kind}_s{scheme}_r{recon}_f{flux}_b{bp}"
+                    with tempfile.TemporaryDirectory() as tmp:
+                        prefix, _ = run_ref(["rhs", 0, nx, ny, 2, 0.4, scheme, bp, flux, recon, 0.0, *params],
+                                            tmp, state=U0)
+                        out[tag + "_rhs"] = load(prefix, "_rhs.bin", (4, ny, nx))
+                        if flux == 1:
+                            out[tag + "_faces"] = load(prefix, "_faces.bin", (ny, nx, 64))
+                        out[tag + "_thflag"] = load(prefix, "_thetaflag.bin", (ny, nx))
+    np.savez_compressed(os.path.join(HERE, "solver_rhs_exp.npz"), **out)
+    print("WENO-EXP / primitive3 rhs fixtures written")
+
+
+RUNS_EXP = [
+    ("smooth32_exp_llf_cons_rk3", 0, 32, 32, 2, 0.4, 2, 0, 0, 0, "run", 0.05),
+    ("smooth32_exp_hllc_char_bp_rk3", 0, 32, 32, 2, 0.4, 2, 1, 1, 1, "run", 0.05),
+    ("smooth32_prim3_hllc_bp_rk3", 0, 32, 32, 2, 0.4, 1, 1, 1, 4, "run", 0.05),
+    ("sharp32_exp_hllc_cons_bp_rk3", 2, 32, 32, 2, 0.4, 2, 1, 1, 0, "steps", 20),
+    ("vortex32_prim3_llf_bp_rk2", 1, 32, 32, 1, 0.4, 1, 1, 0, 4, "run", 0.003),
+]
+
+
+def make_runs_exp():
+    out = {}
+    for name, tc, nx, ny, rk, cfl, scheme, bp, flux, recon, mode, tend in RUNS_EXP:
+        with tempfile.TemporaryDirectory() as tmp:
+            prefix, so = run_ref([mode, tc, nx, ny, rk, cfl, scheme, bp, flux, recon, tend], tmp)
+            out[name + "_init"] = load(prefix, "_init.bin", (4, ny, nx))
+            out[name + "_final"] = load(prefix, "_final.bin", (4, ny, nx))
+            out[name + "_dts"] = load(prefix, "_dts.bin")
+            out[name + "_opts"] = np.array([tc, nx, ny, rk, cfl, scheme, bp, flux, recon, 1 if mode == "steps" else 0,
+                                            tend], dtype=np.float64)
+            print(name, so.strip().splitlines()[-1])
+    np.savez_compressed(os.path.join(HERE, "solver_runs_exp.npz"), **out)
+    print("WENO-EXP / primitive3 run fixtures written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", default=None)
@@ -307,7 +407,11 @@ def main():
         sys.exit("oracle/_ref not built: run oracle/build_ref.sh first")
     rng = np.random.default_rng(20261203)
     jobs = {"point": lambda: make_point(rng), "rhs": lambda: make_rhs(rng), "theta": lambda: make_theta(rng),
-            "runs": make_runs}
+            "runs": make_runs,
+            # separate generators, so the fixtures above stay unchanged
+            "pointexp": lambda: make_point_exp(np.random.default_rng(7007)),
+            "rhsexp": lambda: make_rhs_exp(np.random.default_rng(7008)),
+            "runsexp": make_runs_exp}
     for k, f in jobs.items():
         if a.only is None or k in a.only:
             f()

@@ -126,3 +126,38 @@ def test_theta_matches_reference_bitwise_and_bounds():
                 lim = got * (faces[:, r, :] - U[r]) + U[r]
                 # the limited value theta*(v-u)+u is rounded at the scale of u
                 assert lim.min() >= min(1e-12, U[r]) - 8 * np.finfo(float).eps * abs(U[r])
+
+
+EXP = golden("point_wenoexp.npz")
+
+
+def test_weno_exp_and_thinc_match_reference_bitwise():
+    S, dxs = EXP["stencils"], EXP["dx"]
+    for k, dx in enumerate(dxs):
+        for g in range(4):
+            e = np.array([orc.weno_exp(s, g, dx) for s in S])
+            t = np.array([orc.weno_exp_thinc(s, g, dx) for s in S])
+            np.testing.assert_array_equal(e, EXP["exp"][k, :, g])
+            np.testing.assert_array_equal(t, EXP["thinc"][k, :, g])
+
+
+def test_weno7_dispatch_uses_inner_stencil_for_js_and_z():
+    rng = np.random.default_rng(3)
+    for s in rng.standard_normal((50, 7)):
+        for g in range(4):
+            for scheme in (0, 1):
+                assert orc.weno7(scheme, s, g, 0.1) == orc.weno(scheme, s[1:6], g)
+            assert orc.weno7(2, s, g, 0.1) == orc.weno_exp(s, g, 0.1)
+
+
+def test_weno_exp_reproduces_constants_and_thinc_stays_within_neighbours():
+    for c in (0.0, 0.3, 1.0, -2.5):
+        for g in range(4):
+            assert abs(orc.weno_exp(np.full(7, c), g, 0.1) - c) <= 1e-15 * max(1.0, abs(c))
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        z = np.sort(rng.uniform(1e-5, 1 - 1e-5, 7))[:: rng.choice([-1, 1])]
+        for g in range(4):
+            v = orc.weno_exp_thinc(z, g, 0.05)
+            lo, hi = min(z[2], z[4]), max(z[2], z[4])
+            assert lo - 1e-12 <= v <= hi + 1e-12  # THINC on a monotone stencil: bounded by the neighbours

@@ -6,8 +6,10 @@ import pytest
 import oracle as orc
 from conftest import golden
 
-RHS = golden("solver_rhs.npz")
-RUNS = golden("solver_runs.npz")
+# SCHEME 0/1 x RECONSTRUCTIONTYPE 0-3, and SCHEME 2 (WENO-EXP) with RECONSTRUCTIONTYPE 4
+# (primitive3, THINC on z1) in their own files
+RHS_FILES = {"base": golden("solver_rhs.npz"), "exp": golden("solver_rhs_exp.npz")}
+RUN_FILES = {"base": golden("solver_runs.npz"), "exp": golden("solver_runs_exp.npz")}
 
 # reference test-case parameters (include/Equations.hpp:490-628): domain, rho10, rho20, c1, c2, preb
 CASE = {
@@ -18,13 +20,15 @@ CASE = {
 
 
 def rhs_cases():
-    for key in RHS.files:
-        if key.endswith("_rhs"):
-            yield key[: -len("_rhs")]
+    for src, RHS in RHS_FILES.items():
+        for key in sorted(RHS.files):
+            if key.endswith("_rhs"):
+                yield src, key[: -len("_rhs")]
 
 
-@pytest.mark.parametrize("tag", sorted(rhs_cases()))
-def test_assemble_rhs_matches_reference(tag):
+@pytest.mark.parametrize("src,tag", list(rhs_cases()))
+def test_assemble_rhs_matches_reference(src, tag):
+    RHS = RHS_FILES[src]
     kind, s, r, f, b = tag.split("_")
     params = RHS[kind + "_params"]
     U = RHS[kind + "_U"]
@@ -43,11 +47,12 @@ def test_assemble_rhs_matches_reference(tag):
 
 
 def run_names():
-    return sorted(k[: -len("_opts")] for k in RUNS.files if k.endswith("_opts"))
+    return [(src, k[: -len("_opts")]) for src, R in RUN_FILES.items() for k in sorted(R.files) if k.endswith("_opts")]
 
 
-@pytest.mark.parametrize("name", run_names())
-def test_time_loop_matches_reference(name):
+@pytest.mark.parametrize("src,name", run_names())
+def test_time_loop_matches_reference(src, name):
+    RUNS = RUN_FILES[src]
     tc, nx, ny, rk, cfl, scheme, bp, flux, recon, fixed, tend = RUNS[name + "_opts"]
     dom, rho10, rho20, c1, c2, preb = CASE[int(tc)]
     o = orc.Oracle(int(nx), int(ny), domain=dom, scheme=int(scheme), recon=int(recon), flux=int(flux),
