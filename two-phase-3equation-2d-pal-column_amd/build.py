@@ -23,7 +23,9 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 # translation units and their contraction policy
 UNITS = [
     ("wenofv_kernels_fp64.hip", ["-ffp-contract=off"]),
+    ("wenofv_kernels_fp64_r3.hip", ["-ffp-contract=off"]),
     ("wenofv_kernels_mixed.hip", ["-ffp-contract=fast"]),
+    ("wenofv_kernels_mixed_r3.hip", ["-ffp-contract=fast"]),
     ("wenofv_control.hip", ["-ffp-contract=off"]),
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),
     ("wenofv_tune.cpp", ["-ffp-contract=off"]),

@@ -49,8 +49,6 @@ void CWENOFV::initializeSolver() {
     m_precision = option.count("PRECISION") ? std::stoi(option["PRECISION"]) : 0;
     m_deltaX = (equation->xR - equation->xL) / m_worldElemNumX;
     m_deltaY = (equation->yR - equation->yL) / m_worldElemNumY;
-    if (m_scheme == WENOEXP || m_reconstructionType == PRIMITIVE3)
-        throw std::invalid_argument("SCHEME=2 (WENO-EXP) and RECONSTRUCTIONTYPE=4 are not provided by this build");
 
     wfv_options o{};
     o.elem_num_x = m_worldElemNumX;

@@ -7,10 +7,11 @@
 // face quadrature, divergence and the RK combination; the last stage also reduces the
 // next CFL time step.
 //
-// Work decomposition.  A wavefront (64 lanes) owns a strip of 58 columns and marches up
-// `ty` rows.  Lane l holds column x0-3+l.  For each row r it
-//   1. keeps cell averages of rows r-2..r+2 of its column in registers (the periodic or
-//      wall ghost map is applied by the loader, so no ghost cells exist in memory),
+// Work decomposition.  A wavefront (64 lanes) owns a strip of 58 columns (56 for the 7-point
+// forms, see Stencil) and marches up `ty` rows.  Lane l holds column x0-3+l (x0-4+l).  For
+// each row r it
+//   1. keeps cell averages of rows r-2..r+2 (r-3..r+3) of its column in registers (the
+//      periodic or wall ghost map is applied by the loader, so no ghost cells exist in memory),
 //   2. reconstructs its column in y at the 4 Gauss-Lobatto nodes (one stencil per column
 //      and row, shared by the 5 cells whose x-stencil uses it) and publishes the 16
 //      values in LDS,
@@ -20,8 +21,8 @@
 //      top face values kept in registers, and on the face east of its cell with the
 //      neighbour lane's west face values exchanged through LDS,
 //   5. completes the RHS of the cell of row r-1, applies the RK combination and stores.
-// Lanes 3..60 own output cells; lanes 2 and 61 recompute the neighbouring strips'
-// edge cells, whose face values the strip's boundary fluxes need.
+// Lanes 3..60 (4..59) own output cells; lanes 2 and 61 (3 and 60) recompute the neighbouring
+// strips' edge cells, whose face values the strip's boundary fluxes need.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,8 +34,7 @@
 
 namespace wfv {
 
-constexpr int LANES = 64;
-constexpr int STRIP_W = 58;  // owned columns per wavefront
+constexpr int LANES = WAVE_LANES;
 
 WFV_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -227,19 +227,61 @@ WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceS
 }
 
 // reconstruction arithmetic of a policy and form (see stage_body)
-template <class PR, int RECON>
-using ReconT = typename std::conditional<RECON == 1, double, typename PR::W>::type;
+template <class PR, int RECON, int SCHEME>
+using ReconT = typename std::conditional<RECON == 1 || Stencil<SCHEME, RECON>::R == 3, double, typename PR::W>::type;
+
+// the 4 Gauss-Lobatto node values of variable v from a 7-point stencil: WENO-EXP (SCHEME 2),
+// or for primitive3 THINC on z1 and WENO-Z on p, u, v (src/WENOFV.cpp:1287-1329)
+template <int SCHEME, int RECON>
+WFV_DEV void recon7(const double s[7], int v, double dx4, double out[4]) {
+    if (RECON == 4 && v != 0) {
+        Weno5<double, 1> st(s[1], s[2], s[3], s[4], s[5]);
+        out[0] = st.template node<0>(); out[1] = st.template node<1>();
+        out[2] = st.template node<2>(); out[3] = st.template node<3>();
+    } else if (RECON == 4) {
+        ExpThinc7 st(s, dx4);
+        out[0] = st.template node<0>(); out[1] = st.template node<1>();
+        out[2] = st.template node<2>(); out[3] = st.template node<3>();
+    } else {
+        WenoExp7 st(s, dx4);
+        out[0] = st.template node<0>(); out[1] = st.template node<1>();
+        out[2] = st.template node<2>(); out[3] = st.template node<3>();
+    }
+}
+// the same at the 12 boundary nodes' x positions: nodes 0 and 3 always, 1 and 2 if `all`
+template <int SCHEME, int RECON>
+WFV_DEV void recon7_x(const double s[7], int v, double dx4, bool all, double out[4]) {
+    if (RECON == 4 && v != 0) {
+        Weno5<double, 1> st(s[1], s[2], s[3], s[4], s[5]);
+        out[0] = st.template node<0>(); out[3] = st.template node<3>();
+        if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
+    } else if (RECON == 4) {
+        ExpThinc7 st(s, dx4);
+        out[0] = st.template node<0>(); out[3] = st.template node<3>();
+        if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
+    } else {
+        WenoExp7 st(s, dx4);
+        out[0] = st.template node<0>(); out[3] = st.template node<3>();
+        if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
+    }
+}
 
 // One wavefront's strip of one stage (the whole algorithm; see the file comment).
 template <class PR, int RECON, int SCHEME, int FLUX>
-WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, RECON>> &L) {
+WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     // Reconstruction arithmetic.  The characteristic form projects with eigenvector
     // matrices whose condition number reaches ~1e8 in near-pure phases (z1 -> 0 or 1), which
     // would amplify FP32 rounding of the characteristic values to O(1); the mixed policy
-    // therefore keeps that form's reconstruction in FP64 (its fluxes stay FP32).
-    using W = ReconT<PR, RECON>;
+    // therefore keeps that form's reconstruction in FP64 (its fluxes stay FP32), and so it
+    // does for the 7-point forms.  DEV: the mixed 5-point forms reconstruct deviations from
+    // the cell's mean; the 7-point forms reconstruct the values themselves (THINC tests the
+    // absolute volume fraction).
+    using W = ReconT<PR, RECON, SCHEME>;
     using FS = FaceSide<PR>;
+    using ST = Stencil<SCHEME, RECON>;
+    constexpr int R = ST::R, NW = ST::NW, HALO = ST::HALO, SW = ST::SW;
     constexpr bool EX = PR::exact;
+    constexpr bool DEV = !EX && R == 2;
     const int lane = threadIdx.x & (LANES - 1);
 
     if (strip >= A.strips_x * A.strips_y) return;
@@ -247,16 +289,16 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     const int sx = strip % A.strips_x, sy = strip / A.strips_x;
     const int nx = A.nx, ny = A.ny;
     const size_t nc = (size_t)nx * ny;
-    const int x0 = sx * STRIP_W;
+    const int x0 = sx * SW;
     const int y0 = sy * A.ty;
     const int y1 = min(y0 + A.ty, ny);
     const int nrow = y1 - y0;
-    const int xc = x0 - 3 + lane;
+    const int xc = x0 - HALO + lane;
     bool mu;
     const int cx = map_index(xc, nx, A.bc, mu);
-    const bool owned = lane >= 3 && lane < 3 + STRIP_W && xc < nx;
-    const bool recon_lane = lane >= 2 && lane <= 61;
-    const bool xface_lane = lane >= 2 && lane <= 60;
+    const bool owned = lane >= HALO && lane < HALO + SW && xc < nx;
+    const bool recon_lane = lane >= R && lane <= LANES - 1 - R;
+    const bool xface_lane = lane >= R && lane <= LANES - 2 - R;
     const Phys P = A.P;
     const bool bp = A.bp != 0;
 
@@ -273,18 +315,18 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr double omega1 = 0.1;
     constexpr double omegaR = 1.0 - 2 * omega1;
 
-    // register window: cell averages of rows r-2..r+2 of this lane's column, and their
-    // reconstructed (primitive) variables for RECON 2, 3
-    double win[5][4];
-    double pwin[5][4];
+    // register window: cell averages of rows r-R..r+R of this lane's column, and their
+    // reconstructed (primitive) variables for RECON 2, 3, 4
+    double win[NW][4];
+    double pwin[NW][4];
     auto load_row = [&](int row, double out[4]) {
         bool mv;
         const int cy = map_index(row, ny, A.bc, mv);
         load_cell<double>(A.U, nc, (size_t)cy * nx + cx, mu, mv, out);
     };
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        load_row(y0 - 3 + k, win[k]);
+    for (int k = 0; k < NW; ++k) {
+        load_row(y0 - 1 - R + k, win[k]);
         if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[k], pwin[k]);
     }
 
@@ -304,27 +346,62 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         const int r = y0 - 1 + it;
         if (it > 0) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < NW - 1; ++k)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     win[k][v] = win[k + 1][v];
                     if constexpr (RECON >= 2) pwin[k][v] = pwin[k + 1][v];
                 }
-            load_row(r + 2, win[4]);
-            if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[4], pwin[4]);
+            load_row(r + R, win[NW - 1]);
+            if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[NW - 1], pwin[NW - 1]);
         }
-        const double *Uc = win[2];                          // cell (xc, r)
-        const double *Bc = (RECON >= 2) ? pwin[2] : win[2];  // its reconstructed variables
+        const double *Uc = win[R];                          // cell (xc, r)
+        const double *Bc = (RECON >= 2) ? pwin[R] : win[R];  // its reconstructed variables
 
         // ---- 1. y-direction reconstruction of this lane's column at row r
         //      (getFlux_* step 3, src/WENOFV.cpp:670-687; characteristic 796-833)
         {
-            auto sv = [&](int t, int v) -> W {  // stencil value (mixed: deviation from row r)
+            auto sv = [&](int t, int v) -> W {  // stencil value (DEV: deviation from row r)
                 const double x = (RECON >= 2) ? pwin[t][v] : win[t][v];
-                if constexpr (EX) return x;
+                if constexpr (!DEV) return x;
                 else return W(x - Bc[v]);
             };
-            if constexpr (RECON == 1) {
+            if constexpr (R == 3 && RECON == 1) {
+                double Lm[16], Rm[16], pc[7][4], yc[4][4], yv[4];
+                eigen_LR<double, false>(P, Uc, Lm, Rm);
+#pragma unroll
+                for (int t = 0; t < 7; ++t) {
+                    double u4[4] = {sv(t, 0), sv(t, 1), sv(t, 2), sv(t, 3)};
+                    matvec(Lm, u4, pc[t]);
+                }
+#pragma unroll 1
+                for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
+                    const double s7[7] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0], pc[5][0], pc[6][0]};
+                    double o[4];
+                    recon7<SCHEME, RECON>(s7, k, A.dx4, o);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) rotate_push(yc[g], o[g]);
+#pragma unroll
+                    for (int t = 0; t < 7; ++t) rotate_left(pc[t]);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    matvec(Rm, yc[g], yv);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) L.y[g * 4 + v][lane] = yv[v];
+                }
+            } else if constexpr (R == 3) {
+#pragma unroll 1
+                for (int v = 0; v < 4; ++v) {
+                    double s7[7];
+#pragma unroll
+                    for (int t = 0; t < 7; ++t) s7[t] = sv(t, v);
+                    double o[4];
+                    recon7<SCHEME, RECON>(s7, v, A.dx4, o);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) L.y[g * 4 + v][lane] = o[g];
+                }
+            } else if constexpr (RECON == 1) {
                 W Lm[16], Rm[16], pc[5][4], yc[4][4], yv[4];
                 eigen_LR<W, false>(P, Uc, Lm, Rm);
 #pragma unroll
@@ -367,7 +444,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     for (int t = 0; t < 5; ++t) rotate_left(sw[t]);
                 }
             }
-            if constexpr (!EX) {
+            if constexpr (DEV) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) L.base[v][lane] = Bc[v];
             }
@@ -388,14 +465,68 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             }
         double theta = 1.0;
         if (recon_lane) {
-            // x-stencil value of node row gy, variable v, offset t (mixed: deviation from Bc)
+            // x-stencil value of node row gy, variable v, offset t (DEV: deviation from Bc)
             auto xv = [&](int gy, int v, int t) -> W {
-                const int l = lane - 2 + t;
-                if constexpr (EX) return L.y[gy * 4 + v][l];
+                const int l = lane - R + t;
+                if constexpr (!DEV) return L.y[gy * 4 + v][l];
                 else return W(L.base[v][l] - Bc[v]) + L.y[gy * 4 + v][l];
             };
             W ug[4][4][4];  // Ugp[gx][gy][v] at the 12 boundary nodes
-            if constexpr (RECON == 1) {
+            if constexpr (R == 3 && RECON == 1) {
+                double Lm[16], Rm[16];
+                eigen_LR<double, true>(P, Uc, Lm, Rm);
+#pragma unroll 1
+                for (int gy = 0; gy < 4; ++gy) {  // node row gy sits in slot 0 of ug[.][0]
+                    double pc[7][4];
+#pragma unroll
+                    for (int t = 0; t < 7; ++t) {
+                        double y4[4] = {xv(gy, 0, t), xv(gy, 1, t), xv(gy, 2, t), xv(gy, 3, t)};
+                        matvec(Lm, y4, pc[t]);
+                    }
+                    const bool all = gy == 0 || gy == 3;
+                    double uc[4][4];  // [gx][k]
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        const double s7[7] = {pc[0][k], pc[1][k], pc[2][k], pc[3][k], pc[4][k], pc[5][k], pc[6][k]};
+                        double o[4] = {0.0, 0.0, 0.0, 0.0};
+                        recon7_x<SCHEME, RECON>(s7, k, A.dx4, all, o);
+#pragma unroll
+                        for (int gx = 0; gx < 4; ++gx) uc[gx][k] = o[gx];
+                    }
+#pragma unroll
+                    for (int gx = 0; gx < 4; ++gx) {
+                        double o[4];
+                        matvec(Rm, uc[gx], o);
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) rotate_push(ug[gx][v], o[v]);
+                    }
+                }
+                // ug[gx][v][gy] -> ug[gx][gy][v]
+#pragma unroll
+                for (int gx = 0; gx < 4; ++gx)
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = a + 1; b < 4; ++b) {
+                            const W t = ug[gx][a][b];
+                            ug[gx][a][b] = ug[gx][b][a];
+                            ug[gx][b][a] = t;
+                        }
+            } else if constexpr (R == 3) {
+#pragma unroll 1
+                for (int v = 0; v < 4; ++v) {  // variable v sits in slot 3 after the loop
+#pragma unroll
+                    for (int gy = 0; gy < 4; ++gy) {
+                        double s7[7];
+#pragma unroll
+                        for (int t = 0; t < 7; ++t) s7[t] = xv(gy, v, t);
+                        double o[4] = {0.0, 0.0, 0.0, 0.0};
+                        recon7_x<SCHEME, RECON>(s7, v, A.dx4, gy == 0 || gy == 3, o);
+#pragma unroll
+                        for (int gx = 0; gx < 4; ++gx) rotate_push(ug[gx][gy], o[gx]);
+                    }
+                }
+            } else if constexpr (RECON == 1) {
                 W Lm[16], Rm[16];
                 eigen_LR<W, true>(P, Uc, Lm, Rm);
 #pragma unroll
@@ -469,14 +600,17 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 if constexpr (RECON >= 2) {
                     double pv[4];
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) pv[v] = EX ? double(w[v]) : Bc[v] + double(w[v]);
+                    for (int v = 0; v < 4; ++v) pv[v] = DEV ? Bc[v] + double(w[v]) : double(w[v]);
                     double cv[4];
                     from_prim<double, double, RECON>(P, pv, cv);
 #pragma unroll
                     for (int v = 0; v < 4; ++v) f.set(v, g, cv[v], Uc[v]);
-                } else {
+                } else if constexpr (DEV || EX) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) f.set_recon(v, g, w[v], Uc[v]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f.set(v, g, double(w[v]), Uc[v]);
                 }
             };
 #pragma unroll
@@ -626,7 +760,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 const int rr = r - 1;
                 const double dyp = A.dyC[rr];
                 const double idyp = EX ? 0.0 : 1.0 / dyp;
-                const double *Ucur = win[1];
+                const double *Ucur = win[R - 1];
                 double Ub[4];
                 if (A.Un) {
                     load_cell<double>(A.Un, nc, (size_t)rr * nx + xc, false, false, Ub);
@@ -694,7 +828,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 // single domain: one wavefront per strip
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {
-    __shared__ WaveLds<ReconT<PR, RECON>> L;
+    __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
     stage_body<PR, RECON, SCHEME, FLUX>(A, blockIdx.x, L);
 }
 
@@ -706,7 +840,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // parameters, RK registers and device-side time step.  PERSIST: a grid smaller than the
 // item list strides over it (a persistent launch); otherwise one wavefront per item.
 template <class PR, int RECON, int SCHEME, int FLUX>
-WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON>> &L) {
+WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     const int i = B.item_inst[item];
     const BatchInst &I = B.inst[i];
     StepCtl *ctl = I.ctl;
@@ -727,6 +861,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.dX = I.dX;
     A.dY = I.dY;
     A.cfl = I.cfl;
+    A.dx4 = I.dx4;
     A.gravity = I.gravity;
     A.nx = I.nx;
     A.ny = I.ny;
@@ -746,7 +881,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_stage_kernel(
     BatchStageArgs B) {
-    __shared__ WaveLds<ReconT<PR, RECON>> L;
+    __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
     if constexpr (PERSIST) {
         for (int item = blockIdx.x; item < B.total_items; item += gridDim.x)
             batch_item<PR, RECON, SCHEME, FLUX>(B, item, L);

@@ -53,7 +53,9 @@ StageArgs base_args(wfv_ctx *c) {
     A.flux = c->opt.riemann_flux;
     A.bp = c->opt.pp_limiter;
     A.ty = c->ty;
-    A.strips_x = (c->nx + strip_width() - 1) / strip_width();
+    const int sw = strip_width(c->opt.scheme, c->opt.reconstruction);
+    A.strips_x = (c->nx + sw - 1) / sw;
+    A.dx4 = std::pow(c->dX, 4);  // WENO-EXP's threshold, as the reference's pow(m_deltaX, 4)
     A.strips_y = (c->ny + c->ty - 1) / c->ty;
     A.flags = &c->d_ctl->flags;
     return A;
@@ -128,9 +130,9 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     if (opt->elem_num_x < 1 || opt->elem_num_y < 1) why = "ELEMNUMX/ELEMNUMY must be positive";
     else if (opt->elem_num_x < 3 || opt->elem_num_y < 3) why = "grids need at least 3 cells per direction";
     else if (opt->rk_method < 0 || opt->rk_method > 2) why = "RKMETH must be 0, 1 or 2";
-    else if (opt->scheme < 0 || opt->scheme > 1) why = "SCHEME must be 0 (WENO-JS) or 1 (WENO-Z)";
+    else if (opt->scheme < 0 || opt->scheme > 2) why = "SCHEME must be 0 (WENO-JS), 1 (WENO-Z) or 2 (WENO-EXP)";
     else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
-    else if (opt->reconstruction < 0 || opt->reconstruction > 3) why = "RECONSTRUCTIONTYPE must be 0..3";
+    else if (opt->reconstruction < 0 || opt->reconstruction > 4) why = "RECONSTRUCTIONTYPE must be 0..4";
     else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 (fp64) or 1 (mixed)";
     else if (opt->boundary < 0 || opt->boundary > 1) why = "boundary must be 0 (periodic) or 1 (walls)";
     else if (!(opt->cfl > 0)) why = "CFL must be positive";
@@ -408,7 +410,8 @@ int batch_layout(wfv_batch *b) {
     for (int i : order) {
         BatchInst &I = b->hinst[i];
         I.ty = b->ty[i];
-        I.strips_x = (I.nx + strip_width() - 1) / strip_width();
+        const int sw = strip_width(b->opt.scheme, b->opt.reconstruction);
+        I.strips_x = (I.nx + sw - 1) / sw;
         I.strips_y = (I.ny + I.ty - 1) / I.ty;
         I.item_begin = (int)item_inst.size();
         for (int k = 0; k < I.strips_x * I.strips_y; ++k) item_inst.push_back(i);
@@ -475,9 +478,9 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     *out = nullptr;
     std::string why;
     if (opt->rk_method < 0 || opt->rk_method > 2) why = "RKMETH must be 0, 1 or 2";
-    else if (opt->scheme < 0 || opt->scheme > 1) why = "SCHEME must be 0 or 1";
+    else if (opt->scheme < 0 || opt->scheme > 2) why = "SCHEME must be 0, 1 or 2";
     else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
-    else if (opt->reconstruction < 0 || opt->reconstruction > 3) why = "RECONSTRUCTIONTYPE must be 0..3";
+    else if (opt->reconstruction < 0 || opt->reconstruction > 4) why = "RECONSTRUCTIONTYPE must be 0..4";
     else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 or 1";
     else if (!(opt->cfl > 0)) why = "CFL must be positive";
     for (int i = 0; i < n && why.empty(); ++i) {
@@ -550,6 +553,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         I.P.q = in.cs.rho20 * I.P.c2_2 - in.cs.rho10 * I.P.c1_2;
         I.dX = (in.cs.xR - in.cs.xL) / in.nx;
         I.dY = (in.cs.yR - in.cs.yL) / in.ny;
+        I.dx4 = std::pow(I.dX, 4);
         I.cfl = opt->cfl;
         I.gravity = in.cs.gravity;
         I.nx = in.nx;

@@ -182,6 +182,8 @@ hipError_t launch_step_control(StepCtl *S, hipStream_t s) {
     return hipGetLastError();
 }
 
-int strip_width() { return 58; }
+int strip_width(int scheme, int recon) {
+    return (scheme == 2 || recon == 4) ? Stencil<2, 0>::SW : Stencil<0, 0>::SW;
+}
 
 }  // namespace wfv

@@ -385,4 +385,156 @@ struct Weno5 {
     }
 };
 
+// ------------------------------------------------------------------ 7-point kernels
+// WENO-EXP (include/Utility.hpp:291-472) and the THINC kernel of the primitive3 form
+// (include/Utility.hpp:473-663), in double.  s = {u_mmm, u_mm, u_m, u, u_p, u_pp, u_ppp}.
+
+// x^3 rounded once (the reference's pow(x, 3)): x^2 = p + e exactly, p x = hi + lo exactly
+WFV_DEV double cube_rn(double x) {
+    const double p = x * x;
+    const double e = fma(x, x, -p);
+    const double hi = p * x;
+    const double lo = fma(p, x, -hi);
+    return hi + (lo + e * x);
+}
+
+// fitted linear weights, cubic in l2h2 (include/Utility.hpp:347-380); pow(l, 2) is l*l
+template <int GP>
+WFV_DEV void wexp_d(double l, double &d0, double &d1, double &d2) {
+    const double l2 = l * l, l3 = cube_rn(l);
+    if constexpr (GP == 0) {
+        d0 = 0.3 - 0.057142858 * l + 0.0071428571 * l2 - 0.00077200577 * l3;
+        d1 = 0.6 + 0.078571429 * l - 0.0101190476 * l2 + 0.00112012987 * l3;
+        d2 = 0.1 - 0.021428571 * l + 0.0029761905 * l2 - 0.00034812410 * l3;
+    } else if constexpr (GP == 1) {
+        d0 = 0.16108042 - 0.035187607 * l + 0.0049466098 * l2 - 0.00058211525 * l3;
+        d1 = 0.58636364 + 0.087692641 * l - 0.0120401515 * l2 + 0.00139436871 * l3;
+        d2 = 0.25255594 - 0.052505034 * l + 0.0070935417 * l2 - 0.00081225346 * l3;
+    } else if constexpr (GP == 2) {
+        d0 = 0.25255594 - 0.052505034 * l + 0.0070935417 * l2 - 0.00081225346 * l3;
+        d1 = 0.58636364 + 0.087692641 * l - 0.0120401515 * l2 + 0.00139436871 * l3;
+        d2 = 0.16108042 - 0.035187607 * l + 0.0049466098 * l2 - 0.00058211525 * l3;
+    } else {
+        d0 = 0.1 - 0.021428571 * l + 0.0029761905 * l2 - 0.00034812410 * l3;
+        d1 = 0.6 + 0.078571429 * l - 0.0101190476 * l2 + 0.00112012987 * l3;
+        d2 = 0.3 - 0.057142858 * l + 0.0071428571 * l2 - 0.00077200577 * l3;
+    }
+}
+
+// l2h2 = dif5/dif3 of the 6 cells biased toward the node (gp < 2: u_mmm..u_pp; else
+// u_mm..u_ppp), 0 where |dif3| < dx^4 (include/Utility.hpp:297-321)
+WFV_DEV double wexp_l2h2(const double *t, double dx4, bool clamp) {
+    const double dif5 = (-t[0] + 5.0 * t[1] - 10.0 * t[2] + 10.0 * t[3] - 5.0 * t[4] + t[5]);
+    const double dif3 = (t[0] - 11.0 * t[1] + 28.0 * t[2] - 28.0 * t[3] + 11.0 * t[4] - t[5]) / 6.0;
+    if (fabs(dif3) < dx4) return 0.0;
+    double l = dif5 / dif3;
+    if (clamp && fabs(l) > 1) l = 1;
+    return l;
+}
+
+// the inner five cells scaled to [0, 1] by their min and max (include/Utility.hpp:393-401)
+WFV_DEV void wexp_norm(const double s[7], double n[5]) {
+    const double mx = stdMax(stdMax(stdMax(s[1], s[2]), stdMax(s[3], s[4])), s[5]);
+    const double mn = stdMin(stdMin(stdMin(s[1], s[2]), stdMin(s[3], s[4])), s[5]);
+    for (int k = 0; k < 5; ++k) n[k] = (s[1 + k] - mn) / (mx - mn + 1e-20);
+}
+
+// weights -> value: w_k = d_k (1 + a_k), normalised, sum w_k h_k (candidates as WENO5)
+template <int GP>
+WFV_DEV double wexp_combine(const double s[7], double d0, double d1, double d2, double a0, double a1, double a2) {
+    const double h0 = WENO_H[GP][0][0] * s[1] + WENO_H[GP][0][1] * s[2] + WENO_H[GP][0][2] * s[3];
+    const double h1 = WENO_H[GP][1][0] * s[2] + WENO_H[GP][1][1] * s[3] + WENO_H[GP][1][2] * s[4];
+    const double h2 = WENO_H[GP][2][0] * s[3] + WENO_H[GP][2][1] * s[4] + WENO_H[GP][2][2] * s[5];
+    double w0 = d0 * a0, w1 = d1 * a1, w2 = d2 * a2;
+    const double S = w0 + w1 + w2;
+    w0 = w0 / S; w1 = w1 / S; w2 = w2 / S;
+    return w0 * h0 + w1 * h1 + w2 * h2;
+}
+
+// wenoExpthreconstruction: the node-independent parts (indicators, tau) once per stencil
+struct WenoExp7 {
+    const double *s;
+    double dx4, a0, a1, a2, l_lo, l_hi;
+    WFV_DEV WenoExp7(const double *st, double dx4_) : s(st), dx4(dx4_) {
+        double n[5];
+        wexp_norm(s, n);
+        const double theta = 1;
+        const double b0 = theta * 1.0 / 2.0 * fabs(n[0] - 4 * n[1] + 3 * n[2]) + fabs(n[0] - 2 * n[1] + n[2]);
+        const double b1 = theta * 1.0 / 2.0 * fabs(n[1] - n[3]) + fabs(n[1] - 2 * n[2] + n[3]);
+        const double b2 = theta * 1.0 / 2.0 * fabs(3 * n[2] - 4 * n[3] + n[4]) + fabs(n[2] - 2 * n[3] + n[4]);
+        const double tau = n[0] - 4.0 * n[1] + 6.0 * n[2] - 4.0 * n[3] + n[4];
+        const double eps = 1e-16;
+        a0 = 1.0 + 2 * (tau * tau) / ((b0 * b0) + eps);
+        a1 = 1.0 + 2 * (tau * tau) / ((b1 * b1) + eps);
+        a2 = 1.0 + 2 * (tau * tau) / ((b2 * b2) + eps);
+        l_lo = wexp_l2h2(s, dx4, true);
+        l_hi = wexp_l2h2(s + 1, dx4, true);
+    }
+    template <int GP>
+    WFV_DEV double node() const {
+        double d0, d1, d2;
+        wexp_d<GP>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
+        return wexp_combine<GP>(s, d0, d1, d2, a0, a1, a2);
+    }
+};
+
+// wenoExpWithThincReconstruction: a THINC profile in cells with 1e-6 < u < 1-1e-6, else WENO-Z
+// indicators (eps 1e-16) with the unclamped WENO-EXP linear weights
+struct ExpThinc7 {
+    const double *s;
+    bool thinc, flat;
+    double a0, a1, a2, l_lo, l_hi;      // WENO branch
+    double qmin, qmax, theta, xcc;      // THINC branch
+    WFV_DEV ExpThinc7(const double *st, double dx4) : s(st) {
+        const double um = s[2], uc = s[3], up = s[4];
+        thinc = (1e-6 < uc) & (uc < 1 - 1e-6);
+        l_lo = wexp_l2h2(s, dx4, false);
+        l_hi = wexp_l2h2(s + 1, dx4, false);
+        double n[5];
+        wexp_norm(s, n);
+        double t;
+        t = n[0] - 4 * n[1] + 3 * n[2];
+        double b0 = 0.25 * (t * t);
+        t = n[0] - 2 * n[1] + n[2];
+        b0 = b0 + 13.0 / 12.0 * (t * t);
+        t = n[1] - n[3];
+        double b1 = 0.25 * (t * t);
+        t = n[1] - 2 * n[2] + n[3];
+        b1 = b1 + 13.0 / 12.0 * (t * t);
+        t = 3 * n[2] - 4 * n[3] + n[4];
+        double b2 = 0.25 * (t * t);
+        t = n[2] - 2 * n[3] + n[4];
+        b2 = b2 + 13.0 / 12.0 * (t * t);
+        const double eps = 1.0e-16;
+        const double tau = fabs(b0 - b2);
+        a0 = 1 + tau / (b0 + eps);
+        a1 = 1 + tau / (b1 + eps);
+        a2 = 1 + tau / (b2 + eps);
+        // THINC (include/Utility.hpp:633-661); exp(+-beta) as glibc rounds them
+        qmin = stdMin(um, up);
+        qmax = stdMax(um, up) - qmin;
+        theta = (up - uc >= 0) ? 1.0 : -1.0;  // sc_math::Sign (third_party/scTools/MyMath.h:47-50)
+        const double beta = 1.5;
+        const double exp_beta = 0x1.1ed3fe64fc541p+2, exp_mbeta = 0x1.c8f87724b5c1dp-3;
+        const double C = (uc - qmin + 1e-20) / (qmax + 1e-20);
+        const double B = exp(theta * beta * (2.0 * C - 1.0));
+        const double xc = (exp_beta - B) / (B - exp_mbeta);
+        xcc = log(xc) / (2.0 * beta);
+        flat = (uc - um) * (up - uc) < 0;
+    }
+    template <int GP>
+    WFV_DEV double node() const {
+        if (thinc) {
+            // Gauss-Lobatto-4 points (third_party/scTools/GaussQuadratureInRef.hpp:252-262)
+            constexpr double glp[4] = {-1, -0.447213595499957939281834733746, 0.447213595499957939281834733746, 1};
+            if (flat) return s[3];
+            const double beta = 1.5;
+            return 0.5 * qmax * (1 + theta * tanh(beta * ((glp[GP] + 1) / 2 - xcc))) + qmin;
+        }
+        double d0, d1, d2;
+        wexp_d<GP>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
+        return wexp_combine<GP>(s, d0, d1, d2, a0, a1, a2);
+    }
+};
+
 }  // namespace wfv

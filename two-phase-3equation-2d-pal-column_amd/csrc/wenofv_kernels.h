@@ -9,6 +9,20 @@
 
 namespace wfv {
 
+constexpr int WAVE_LANES = 64;
+
+// Stencil of a scheme: radius R = 2 for the 5-point WENO-JS/Z, 3 for the 7-point WENO-EXP
+// (SCHEME 2) and primitive3 (RECONSTRUCTIONTYPE 4).  A strip keeps R+1 halo columns on each
+// side (R for the stencil, one for the neighbouring strips' edge cells), so a wavefront owns
+// 58 or 56 columns.
+template <int SCHEME, int RECON>
+struct Stencil {
+    static constexpr int R = (SCHEME == 2 || RECON == 4) ? 3 : 2;
+    static constexpr int NW = 2 * R + 1;              // stencil width
+    static constexpr int HALO = R + 1;
+    static constexpr int SW = WAVE_LANES - 2 * HALO;  // owned columns per wavefront
+};
+
 // One RK stage: Uo = (a*Un + b*U) + (c*dt)*rhs(U).  Un == nullptr stands for Un = U.
 struct StageArgs {
     const double *U;      // stage input, SoA [4][ny][nx]
@@ -23,6 +37,7 @@ struct StageArgs {
     double a, b, c;       // RK coefficients
     Phys P;
     double dX, dY, cfl, gravity;
+    double dx4;           // pow(dX, 4): WENO-EXP's smoothness threshold, rounded as the host's libm
     int nx, ny, bc, flux, bp;
     int ty;               // rows per strip
     int strips_x, strips_y;
@@ -63,6 +78,7 @@ struct BatchInst {
     StepCtl *ctl;                // this instance's time loop
     Phys P;
     double dX, dY, cfl, gravity;
+    double dx4;                  // pow(dX, 4) (WENO-EXP)
     int nx, ny, bc, ty;
     int strips_x, strips_y;
     int item_begin;              // first item of this instance in the flattened list
@@ -86,6 +102,6 @@ hipError_t launch_batch_step_control(StepCtl *ctl, int n, hipStream_t s);
 hipError_t launch_batch_dt(const BatchInst *inst, int n, int max_cells, hipStream_t s);
 hipError_t launch_dt(const DtArgs &A, hipStream_t s);
 hipError_t launch_step_control(StepCtl *S, hipStream_t s);
-int strip_width();
+int strip_width(int scheme, int recon);  // owned columns per wavefront
 
 }  // namespace wfv
