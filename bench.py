@@ -12,6 +12,8 @@ Workloads
                       limiter, RK3, CFL 0.4, one persistent launch per stage over all
                       instances.  The same batch in all-FP64 gives the baseline throughput
                       and the reference state for the error check (relative L1 per field).
+  settings_cfg        the reference's shipped config/settings.cfg (SMOOTH 100x100, WENO-Z,
+                      characteristic, LLF, CFL 0.5) to the case's final time, --precision
   tanh_interface      128x128 periodic smooth interface, WENO-JS + LLF, to t = 1, --precision
   liquid_column       1024x512 single case (walls, WENO-Z + HLLC + limiter), --precision
   perturbed_column    4096x2048 single case, --precision [--compare]
@@ -45,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="batch256: skip the FP64 baseline run")
     ap.add_argument("--compare", action="store_true", help="single workloads: also run all-FP64 and report errors")
+    ap.add_argument("--scheme", type=int, default=None, help="single workloads: override SCHEME (0 JS, 1 Z, 2 EXP)")
+    ap.add_argument("--recon", type=int, default=None, help="single workloads: override RECONSTRUCTIONTYPE (0-4)")
+    ap.add_argument("--flux", type=int, default=None, help="single workloads: override RIEMANNFLUX (0 LLF, 1 HLLC)")
     ap.add_argument("--tune-cache", action="store_true",
                     help="reuse earlier autotuner choices (WENOFV_TUNING_CACHE) instead of timing the candidates again")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -273,7 +278,19 @@ def run_single(a, torch, wenofv, cases):
     """One case on the single-domain solver.  Fixed-step workloads are timed over their
     steps; tanh_interface runs the reference's time loop to its final time.  --compare also
     runs the case in all-FP64 and reports the field errors of the timed run against it."""
-    makers = {"tanh_interface": lambda: cases.tanh_interface(),
+    def settings_cfg():
+        # the reference's shipped config/settings.cfg: TESTCASE 0 (SMOOTH) 100x100, RK3, CFL 0.5,
+        # WENO-Z, characteristic, LLF, no limiter, to the case's final time; 4-rank cell widths
+        kw, t_end = wenofv.reference_case(0)
+        U0 = wenofv.reference_initial_state(0, 100, 100, geometry_ranks=4)
+        prm = {k: kw[k] for k in ("rho10", "rho20", "sound1", "sound2", "preb")}
+        w = cases.Workload("settings_cfg", 100, 100, kw["domain"], prm, U0, kw["bc"],
+                           dict(scheme=1, recon=1, flux=0, bp=0, rk=2, cfl=0.5, precision=a.precision), t_end=t_end)
+        w.geometry_ranks = 4
+        return w
+
+    makers = {"settings_cfg": settings_cfg,
+              "tanh_interface": lambda: cases.tanh_interface(),
               "liquid_column": lambda: cases.liquid_column(precision=a.precision),
               "perturbed_column": lambda: cases.perturbed_column(precision=a.precision),
               "multi_column": lambda: cases.multi_column(precision=a.precision)}
@@ -282,12 +299,16 @@ def run_single(a, torch, wenofv, cases):
     wl = makers[a.workload]()
     if a.workload == "tanh_interface":
         wl.opts["precision"] = a.precision
+    for key, val in (("scheme", a.scheme), ("recon", a.recon), ("flux", a.flux)):
+        if val is not None:
+            wl.opts[key] = val
     prec = wl.opts.get("precision", 0)
     steps = a.steps or wl.steps or 0
     stream = torch.cuda.Stream()
 
     def run(precision):
-        sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty, precision=precision))
+        sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty, precision=precision,
+                                                             geometry_ranks=getattr(wl, "geometry_ranks", 1)))
         sol.set_stream(stream.cuda_stream)
         sol.set_state(wl.U0)
         tiles = sol.autotune(trials=2, cache=a.tune_cache) if not a.ty else {"rows_per_strip": a.ty}
@@ -308,7 +329,10 @@ def run_single(a, torch, wenofv, cases):
     cells = wl.nx * wl.ny
     bcname = "walls" if wl.bc == 1 else "periodic"
     o = wl.opts
-    desc = (f"{wl.name} {wl.nx}x{wl.ny} {bcname} {'WENO-Z' if o['scheme'] else 'WENO-JS'} "
+    recon_names = ("conservative", "characteristic", "primitive", "primitive2", "primitive3")
+    desc = (f"{wl.name} {wl.nx}x{wl.ny} {bcname} "
+            f"{'primitive3/THINC' if o['recon'] == 4 else ('WENO-JS', 'WENO-Z', 'WENO-EXP')[o['scheme']]} "
+            f"{recon_names[o['recon']] if o['recon'] != 4 else ''} "
             f"{'HLLC' if o['flux'] else 'LLF'}{' BP' if o['bp'] else ''} RK3 CFL{o['cfl']}, "
             + (f"{steps} steps" if steps else f"to t={wl.t_end}"))
     out = {

@@ -26,6 +26,9 @@ extern "C" {
 #define WFV_ERR_NONFINITE_DT 3  /* NaN CFL time step (reference: exit(1), WENOFV.cpp:502-531) */
 #define WFV_ERR_ARG 4           /* null pointer / bad size                                 */
 
+/* ghost layers of a fixed-ghost boundary (boundary 3): the deepest stencil halo */
+#define WFV_GHOST 4
+
 /* error bits accumulated on the device (wfv_get_flags) */
 #define WFV_FLAG_NAN_FLUX 1u   /* a Riemann flux was NaN (reference prints: Equations.hpp:344-359) */
 #define WFV_FLAG_BAD_THETA 2u  /* limiter theta outside [0,1] (reference prints: WENOFV.cpp:1927)  */
@@ -45,7 +48,11 @@ typedef struct wfv_options {
     int reconstruction;  /* RECONSTRUCTIONTYPE: 0 conservative, 1 characteristic,
                             2 primitive (z1, rho1, u, v), 3 primitive2 (z1, p, u, v) */
     int precision;       /* 0: all-FP64, bit-faithful to the reference; 1: mixed precision */
-    int boundary;        /* 0 periodic (the reference's only boundary), 1 reflective walls */
+    int boundary;        /* BOUNDARYTYPE (reference include/Equations.hpp:25-32; its MPI solver
+                            runs only 0): 0 periodic; 1 slip walls (NEUMANN, FREESLIP: mirror,
+                            normal momentum negated); 2 no-slip walls (NoSlip: mirror, both
+                            momenta negated); 3 fixed ghost cells (NEUMANN2, "from the
+                            initial values": wfv_set_ghost_cells) */
     int geometry_ranks;  /* cell widths as the reference's N-rank slab grid builds them (4),
                             or 1 for a single-domain grid */
     int rows_per_strip;  /* rows one wavefront marches (0: tuned default) */
@@ -94,6 +101,12 @@ int wfv_assemble_rhs(wfv_ctx *ctx, double *rhs, double *faces, double *theta);
 int wfv_advance(wfv_ctx *ctx, double t_start, double t_end, int fixed_steps, int *nsteps,
                 double *t_out, double *dts, int max_dts);
 
+/* Ghost cells of a boundary-3 context: Upad is [4][ny+2*WFV_GHOST][nx+2*WFV_GHOST] (the
+   interior part is ignored).  They stay fixed while the solver runs.  No reference
+   counterpart (the reference's setBoundaryAverages2 stops on NEUMANN2,
+   src/WENOFV.cpp:2287-2292). */
+int wfv_set_ghost_cells(wfv_ctx *ctx, const double *Upad);
+
 /* Error bits raised by the kernels since the last call (WFV_FLAG_*); clears them. */
 int wfv_get_flags(wfv_ctx *ctx, unsigned *flags);
 int wfv_synchronize(wfv_ctx *ctx);
@@ -113,7 +126,7 @@ int wfv_set_stream(wfv_ctx *ctx, void *hip_stream);
    ignored); grid size, boundary and physical case are per instance. */
 typedef struct wfv_instance {
     int nx, ny;       /* interior cells */
-    int boundary;     /* 0 periodic, 1 reflective walls */
+    int boundary;     /* as wfv_options.boundary */
     int pad;
     wfv_case cs;
 } wfv_instance;
@@ -125,6 +138,7 @@ void wfv_batch_destroy(wfv_batch *b);
 const char *wfv_batch_last_error(const wfv_batch *b);
 int wfv_batch_set_state(wfv_batch *b, int i, const double *U);  /* instance i, [4][ny][nx] */
 int wfv_batch_get_state(wfv_batch *b, int i, double *U);
+int wfv_batch_set_ghost_cells(wfv_batch *b, int i, const double *Upad);  /* see wfv_set_ghost_cells */
 /* Every instance runs the reference time loop from t = 0 (see wfv_advance): to t_end with
    the last step clipped, or fixed_steps unclipped CFL steps.  nsteps and t_out (optional)
    receive each instance's step count and final time; flags (optional) its error bits. */
@@ -169,6 +183,9 @@ int wfv_reference_case(int testcase, wfv_case *cs, int *boundary, double *t_end)
    Gauss-Legendre cell averages of TwoPhaseEquation::getU0 on the grid of geometry_ranks
    x-slabs; U: [4][ny][nx].  Host only, no device work. */
 int wfv_reference_initial_state(int testcase, int nx, int ny, int geometry_ranks, double *U);
+/* The same on the grid padded by WFV_GHOST cells on every side (cells extended with the
+   same spacing): Upad [4][ny+2*WFV_GHOST][nx+2*WFV_GHOST], the ghost cells of boundary 3. */
+int wfv_reference_initial_state_padded(int testcase, int nx, int ny, int geometry_ranks, double *Upad);
 
 /* Library build identification (kernel variants compiled for gfx950). */
 const char *wfv_version(void);

@@ -29,8 +29,10 @@ class Params(ctypes.Structure):
         ("rho10", ctypes.c_double), ("rho20", ctypes.c_double),
         ("sound1", ctypes.c_double), ("sound2", ctypes.c_double), ("preb", ctypes.c_double),
         ("gravity", ctypes.c_double),
-        ("dxC", _dp), ("dxW", _dp), ("dyC", _dp),
+        ("dxC", _dp), ("dxW", _dp), ("dyC", _dp), ("ghost", _dp),
     ]
+
+GHOST = 4  # ghost layers of a fixed-ghost boundary (bc 3)
 
 
 def build():
@@ -107,13 +109,19 @@ class Oracle:
 
     def __init__(self, nx, ny, domain=(0.0, 1.0, 0.0, 1.0), scheme=0, recon=0, flux=0, bp=0, bc=0,
                  rk=2, cfl=0.4, rho10=1.0, rho20=1.0, sound1=1.0, sound2=1.0, preb=0.0, gravity=0.0,
-                 nranks=4):
+                 nranks=4, ghost=None):
+        """bc: 0 periodic, 1 slip walls, 2 no-slip walls, 3 fixed ghost cells taken from
+        `ghost`, a padded state (4, ny + 2*GHOST, nx + 2*GHOST)."""
         xL, xR, yL, yR = domain
         self.geo = geometry(nx, ny, xL, xR, yL, yR, nranks)
+        self.ghost = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.float64)
+        if bc == 3:
+            assert self.ghost is not None and self.ghost.shape == (4, ny + 2 * GHOST, nx + 2 * GHOST)
         self.p = Params(nx=nx, ny=ny, scheme=scheme, recon=recon, flux=flux, bp=bp, bc=bc, rk=rk, cfl=cfl,
                         dX=(xR - xL) / nx, dY=(yR - yL) / ny, rho10=rho10, rho20=rho20, sound1=sound1,
                         sound2=sound2, preb=preb, gravity=gravity,
-                        dxC=_ptr(self.geo["dxC"]), dxW=_ptr(self.geo["dxW"]), dyC=_ptr(self.geo["dyC"]))
+                        dxC=_ptr(self.geo["dxC"]), dxW=_ptr(self.geo["dxW"]), dyC=_ptr(self.geo["dyC"]),
+                        ghost=_ptr(self.ghost) if self.ghost is not None else None)
         self.nx, self.ny = nx, ny
 
     # point kernels -------------------------------------------------------------------

@@ -540,32 +540,45 @@ static inline double *YP(work *w, int i, int j) { return w->Y + ((size_t)(j + 1)
 static inline double *FP(work *w, int i, int j) { return w->Fc + ((size_t)(j + 1) * (w->nx + 2) + (i + 1)) * 64; }
 
 /* ghost-cell fill: periodic wrap as exchangeGhostCellsValue + setBoundaryAverages2
-   (src/WENOFV.cpp:2175-2295, corners wrap in both directions); reflective walls mirror
-   three layers and negate the normal momentum (no reference counterpart). */
+   (src/WENOFV.cpp:2175-2295, corners wrap in both directions).  The other boundary kinds
+   (include/Equations.hpp:25-32) have no reference implementation: slip walls (1) mirror the
+   interior and negate the normal momentum, no-slip walls (2) negate both momenta, fixed
+   ghost cells (3) come from p->ghost. */
 static void fill_padded(work *w, const double *U) {
     const int nx = w->nx, ny = w->ny;
     const size_t nc = (size_t)nx * ny;
     for (int j = -PAD; j < ny + PAD; ++j) {
         for (int i = -PAD; i < nx + PAD; ++i) {
             int si = i, sj = j;
-            double sgnu = 1.0, sgnv = 1.0;
-            if (w->p->bc == 1) {
-                if (si < 0) { si = -1 - si; sgnu = -1.0; }
-                if (si >= nx) { si = 2 * nx - 1 - si; sgnu = -1.0; }
-                if (sj < 0) { sj = -1 - sj; sgnv = -1.0; }
-                if (sj >= ny) { sj = 2 * ny - 1 - sj; sgnv = -1.0; }
+            double *d = UP(w, i, j);
+            const int bc = w->p->bc;
+            if (bc == 3 && (i < 0 || i >= nx || j < 0 || j >= ny)) {
+                /* fixed ghost cells (NEUMANN2): the padded array the caller provides */
+                const size_t pc = (size_t)(nx + 2 * PAD) * (ny + 2 * PAD);
+                const size_t o = (size_t)(j + PAD) * (nx + 2 * PAD) + (i + PAD);
+                for (int v = 0; v < 4; ++v) d[v] = w->p->ghost[v * pc + o];
+                continue;
+            }
+            int mu = 0, mv = 0;
+            if (bc != 0) {
+                if (si < 0) { si = -1 - si; mu = 1; }
+                if (si >= nx) { si = 2 * nx - 1 - si; mu = 1; }
+                if (sj < 0) { sj = -1 - sj; mv = 1; }
+                if (sj >= ny) { sj = 2 * ny - 1 - sj; mv = 1; }
             } else {
                 si = ((si % nx) + nx) % nx;
                 sj = ((sj % ny) + ny) % ny;
             }
             const size_t c = (size_t)sj * nx + si;
-            double *d = UP(w, i, j);
             d[0] = U[c];
             d[1] = U[nc + c];
             d[2] = U[2 * nc + c];
             d[3] = U[3 * nc + c];
-            if (sgnu < 0) d[2] = -d[2];
-            if (sgnv < 0) d[3] = -d[3];
+            /* slip walls negate the momentum normal to each mirror; no-slip walls (bc 2)
+               negate both components once per mirror */
+            const int nu = bc == 2 ? mu != mv : mu, nv = bc == 2 ? mu != mv : mv;
+            if (nu) d[2] = -d[2];
+            if (nv) d[3] = -d[3];
         }
     }
 }

@@ -39,7 +39,7 @@ typedef struct orc_params {
                           4 primitive3 (THINC on z1) */
     int flux;          /* 0 LLF, 1 HLLC                                          */
     int bp;            /* bound-preserving limiter on partial densities          */
-    int bc;            /* 0 periodic, 1 reflective walls                         */
+    int bc;            /* 0 periodic, 1 slip walls, 2 no-slip walls, 3 fixed ghost cells */
     int rk;            /* 0 SSP-RK1, 1 SSP-RK2, 2 SSP-RK3                         */
     double cfl;
     double dX, dY;     /* global cell sizes (reference m_deltaX, m_deltaY)        */
@@ -49,6 +49,7 @@ typedef struct orc_params {
     const double *dxW; /* [nx] x-width of the west neighbour column, as seen by
                           the rank that owns the column                          */
     const double *dyC; /* [ny] y-width of each row                               */
+    const double *ghost; /* bc 3: padded state [4][ny+8][nx+8] holding the ghost cells */
 } orc_params;
 
 /* error bits returned by the stepping functions */

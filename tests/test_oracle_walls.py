@@ -96,3 +96,49 @@ def test_walls_conserve_phase_mass():
     assert err == 0
     m0, m1 = U[:2].sum(axis=(1, 2)), U1[:2].sum(axis=(1, 2))
     np.testing.assert_allclose(m1, m0, rtol=1e-13)
+
+
+def mirrored_noslip(U):
+    """As mirrored(), with no-slip images: both momenta negated once per mirror."""
+    X = np.flip(U, axis=2).copy()
+    X[2:] *= -1.0
+    top = np.concatenate([U, X], axis=2)
+    Y = np.flip(top, axis=1).copy()
+    Y[2:] *= -1.0
+    return np.concatenate([top, Y], axis=1)
+
+
+@pytest.mark.parametrize("scheme,recon,flux,bp", FORMS + [(2, 0, 1, 1), (1, 4, 0, 1)])
+def test_noslip_rhs_equals_mirrored_periodic_domain(scheme, recon, flux, bp):
+    """No-slip walls (bc 2): the same construction with both momenta negated in the images."""
+    U = smooth_state(NX, NY, LX, LY, seed=7 + recon)
+    opts = dict(scheme=scheme, recon=recon, flux=flux, bp=bp, rk=2, cfl=0.4, nranks=1, **PRM)
+    walls = orc.Oracle(NX, NY, domain=(0.0, LX, 0.0, LY), bc=2, **opts)
+    per = orc.Oracle(2 * NX, 2 * NY, domain=(0.0, 2 * LX, 0.0, 2 * LY), bc=0, **opts)
+    rw, fw, tw, ew = walls.assemble_rhs(U, want_faces=True)
+    rp, fp, tp, ep = per.assemble_rhs(mirrored_noslip(U), want_faces=True)
+    assert ew == ep == 0
+    np.testing.assert_array_equal(rw, rp[:, :NY, :NX])
+    np.testing.assert_array_equal(fw, fp[:NY, :NX])
+    np.testing.assert_array_equal(tw, tp[:NY, :NX])
+
+
+@pytest.mark.parametrize("scheme,recon,flux,bp", FORMS + [(2, 1, 1, 1), (1, 4, 1, 1)])
+def test_fixed_ghost_rhs_equals_embedding_periodic_domain(scheme, recon, flux, bp):
+    """Fixed ghost cells (bc 3, the reference's NEUMANN2): one RHS equals the interior of a
+    larger periodic domain that holds the same values in the WFV_GHOST-deep ring (and
+    anything further out, which no interior stencil reaches)."""
+    G, E = orc.GHOST, 2 * orc.GHOST  # ghost layers; embedding margin
+    big = smooth_state(NX + 2 * E, NY + 2 * E, LX * (NX + 2 * E) / NX, LY * (NY + 2 * E) / NY, seed=11 + recon)
+    U = big[:, E:E + NY, E:E + NX]
+    pad = big[:, E - G:E + NY + G, E - G:E + NX + G]
+    opts = dict(scheme=scheme, recon=recon, flux=flux, bp=bp, rk=2, cfl=0.4, nranks=1, **PRM)
+    fixed = orc.Oracle(NX, NY, domain=(0.0, LX, 0.0, LY), bc=3, ghost=pad, **opts)
+    per = orc.Oracle(NX + 2 * E, NY + 2 * E, domain=(0.0, LX * (NX + 2 * E) / NX, 0.0, LY * (NY + 2 * E) / NY),
+                     bc=0, **opts)
+    rf, ff, tf, ef = fixed.assemble_rhs(U, want_faces=True)
+    rp, fp, tp, ep = per.assemble_rhs(big, want_faces=True)
+    assert ef == 0
+    np.testing.assert_array_equal(rf, rp[:, E:E + NY, E:E + NX])
+    np.testing.assert_array_equal(ff, fp[E:E + NY, E:E + NX])
+    np.testing.assert_array_equal(tf, tp[E:E + NY, E:E + NX])

@@ -25,6 +25,16 @@ void CWENOFV::check(int rc, const char *what) const {
     if (rc != WFV_OK) throw std::runtime_error(std::string(what) + ": " + wfv_last_error(ctx_));
 }
 
+// BOUNDARYTYPE (include/Equations.hpp:25-32) -> wfv_options.boundary
+static int boundary_kind(BOUNDARYTYPE b) {
+    switch (b) {
+    case PERIOD: return 0;
+    case NoSlip: return 2;
+    case NEUMANN2: return 3;
+    default: return 1;  // NEUMANN, FREESLIP: reflective slip walls
+    }
+}
+
 int CWENOFV::stoi_key(const char *key) {
     // a missing key makes the reference's std::stoi("") throw (src/WENOFV.cpp:43-63)
     auto it = option.find(key);
@@ -60,7 +70,7 @@ void CWENOFV::initializeSolver() {
     o.riemann_flux = m_riemannFluxType;
     o.reconstruction = m_reconstructionType;
     o.precision = m_precision;
-    o.boundary = equation->boundaryCondition == PERIOD ? 0 : 1;
+    o.boundary = boundary_kind(equation->boundaryCondition);
     // the reference splits x over exactly 4 ranks and needs ELEMNUMX % 4 == 0
     // (src/WENOFV.cpp:24, 71-76); its per-rank cell widths are reproduced when possible
     o.geometry_ranks = (m_worldElemNumX % 4 == 0) ? 4 : 1;
@@ -92,6 +102,14 @@ void CWENOFV::initializeAve() {
                                       (m_worldElemNumX % 4 == 0) ? 4 : 1, m_Uh.data()),
           "initializeAve");
     check(wfv_set_state(ctx_, m_Uh.data()), "wfv_set_state");
+    if (equation->boundaryCondition == NEUMANN2) {
+        // NEUMANN2 ("boundary from the initial values"): ghost cells hold the initial state
+        std::vector<double> pad(4 * (size_t)(m_worldElemNumX + 2 * WFV_GHOST) * (m_worldElemNumY + 2 * WFV_GHOST));
+        check(wfv_reference_initial_state_padded(m_testcase, m_worldElemNumX, m_worldElemNumY,
+                                                 (m_worldElemNumX % 4 == 0) ? 4 : 1, pad.data()),
+              "initializeAve");
+        check(wfv_set_ghost_cells(ctx_, pad.data()), "wfv_set_ghost_cells");
+    }
 }
 
 double CWENOFV::calculateDeltaT(double) {
@@ -231,8 +249,48 @@ extern "C" int wfv_reference_case(int testcase, wfv_case *cs, int *boundary, dou
     TwoPhaseEquation e;
     e.setEquationParameters((TESTCASETYPE)testcase);
     *cs = wfv_case{e.xL, e.xR, e.yL, e.yR, e.g_rho10, e.g_rho20, e.g_sound1, e.g_sound2, e.g_preb, e.g_gravity};
-    if (boundary) *boundary = e.boundaryCondition == PERIOD ? 0 : 1;
+    if (boundary) *boundary = boundary_kind(e.boundaryCondition);
     if (t_end) *t_end = e.outputTimes.back();
+    return WFV_OK;
+}
+
+// cell averages of getU0 over one cell: 4x4 Gauss-Legendre (src/WENOFV.cpp:232-279)
+static void cell_average(TwoPhaseEquation &e, double xc, double yc, double dx, double dy, double out[4]) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0}, u[4];
+    for (int gi = 0; gi < 4; ++gi)
+        for (int gj = 0; gj < 4; ++gj) {
+            const double gx = xc + 0.5 * dx * GLP[gi];
+            const double gy = yc + 0.5 * dy * GLP[gj];
+            const double wx = 0.5 * dx * GLWT[gi];
+            const double wy = 0.5 * dy * GLWT[gj];
+            e.getU0(gx, gy, u);
+            for (int r = 0; r < 4; ++r) acc[r] += u[r] * wx * wy;
+        }
+    const double area = dx * dy;
+    for (int r = 0; r < 4; ++r) out[r] = acc[r] / area;
+}
+
+extern "C" int wfv_reference_initial_state_padded(int testcase, int nx, int ny, int geometry_ranks, double *Upad) {
+    if (!Upad || testcase < 0 || testcase > 8 || nx < 1 || ny < 1) return WFV_ERR_ARG;
+    const int G = WFV_GHOST, px = nx + 2 * G, py = ny + 2 * G;
+    std::vector<double> U(4 * (size_t)nx * ny);
+    const int rc = wfv_reference_initial_state(testcase, nx, ny, geometry_ranks, U.data());
+    if (rc != WFV_OK) return rc;
+    TwoPhaseEquation e;
+    e.setEquationParameters((TESTCASETYPE)testcase);
+    const double dX = (e.xR - e.xL) / nx, dY = (e.yR - e.yL) / ny;
+    const size_t pc = (size_t)px * py, nc = (size_t)nx * ny;
+    for (int j = -G; j < ny + G; ++j)
+        for (int i = -G; i < nx + G; ++i) {
+            const size_t o = (size_t)(j + G) * px + (i + G);
+            if (i >= 0 && i < nx && j >= 0 && j < ny) {
+                for (int r = 0; r < 4; ++r) Upad[r * pc + o] = U[r * nc + (size_t)j * nx + i];
+                continue;
+            }
+            double a[4];  // ghost cells continue the uniform grid
+            cell_average(e, e.xL + (i + 0.5) * dX, e.yL + (j + 0.5) * dY, dX, dY, a);
+            for (int r = 0; r < 4; ++r) Upad[r * pc + o] = a[r];
+        }
     return WFV_OK;
 }
 

@@ -42,8 +42,10 @@ WFV_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ghost-cell map: periodic wrap (src/WENOFV.cpp:2175-2295) or reflective wall mirror with
-// the normal momentum negated; the result is clamped so that every load stays in bounds.
+// Ghost-cell map of a boundary kind (include/Equations.hpp:25-32): 0 periodic wrap
+// (src/WENOFV.cpp:2175-2295); 1 slip walls (NEUMANN, FREESLIP) and 2 no-slip walls (NoSlip)
+// mirror the interior; 3 (NEUMANN2) reads fixed ghost cells (ghost_offset).  The result is
+// clamped so that every load stays in bounds.
 WFV_DEV int map_index(int i, int n, int bc, bool &mirrored) {
     mirrored = false;
     if (bc == 0) {
@@ -56,11 +58,17 @@ WFV_DEV int map_index(int i, int n, int bc, bool &mirrored) {
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+// A state across mirrors: slip walls negate the momentum normal to each mirror (mu: x, mv:
+// y); no-slip walls negate both components once per mirror.
 template <class T>
-WFV_DEV void load_cell(const double *__restrict__ U, size_t nc, size_t off, bool mu, bool mv, T out[4]) {
+WFV_DEV void load_cell(const double *__restrict__ U, size_t nc, size_t off, bool mu, bool mv, int bc, T out[4]) {
     double a = U[off], b = U[nc + off], c = U[2 * nc + off], d = U[3 * nc + off];
-    out[0] = T(a); out[1] = T(b); out[2] = T(mu ? -c : c); out[3] = T(mv ? -d : d);
+    const bool nu = bc == 2 ? mu != mv : mu;
+    const bool nv = bc == 2 ? mu != mv : mv;
+    out[0] = T(a); out[1] = T(b); out[2] = T(nu ? -c : c); out[3] = T(nv ? -d : d);
 }
+
+
 
 // primitive variables (src/WENOFV.cpp:937-956 PRIMITIVE, 1097-1117 PRIMITIVE2)
 template <class W, int RECON>
@@ -320,9 +328,15 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     double win[NW][4];
     double pwin[NW][4];
     auto load_row = [&](int row, double out[4]) {
+        if (A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny)) {
+            const size_t rs = ring_size(nx, ny), o = ghost_offset(nx, ny, xc, row);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[v] = A.ghost[v * rs + o];
+            return;
+        }
         bool mv;
         const int cy = map_index(row, ny, A.bc, mv);
-        load_cell<double>(A.U, nc, (size_t)cy * nx + cx, mu, mv, out);
+        load_cell<double>(A.U, nc, (size_t)cy * nx + cx, mu, mv, A.bc, out);
     };
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
@@ -763,7 +777,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 const double *Ucur = win[R - 1];
                 double Ub[4];
                 if (A.Un) {
-                    load_cell<double>(A.Un, nc, (size_t)rr * nx + xc, false, false, Ub);
+                    load_cell<double>(A.Un, nc, (size_t)rr * nx + xc, false, false, 0, Ub);
                 } else {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) Ub[v] = Ucur[v];
@@ -862,6 +876,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.dY = I.dY;
     A.cfl = I.cfl;
     A.dx4 = I.dx4;
+    A.ghost = I.ghost;
     A.gravity = I.gravity;
     A.nx = I.nx;
     A.ny = I.ny;

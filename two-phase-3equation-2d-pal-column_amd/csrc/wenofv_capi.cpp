@@ -50,6 +50,7 @@ StageArgs base_args(wfv_ctx *c) {
     A.nx = c->nx;
     A.ny = c->ny;
     A.bc = c->opt.boundary;
+    A.ghost = c->d_ghost;
     A.flux = c->opt.riemann_flux;
     A.bp = c->opt.pp_limiter;
     A.ty = c->ty;
@@ -134,7 +135,8 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
     else if (opt->reconstruction < 0 || opt->reconstruction > 4) why = "RECONSTRUCTIONTYPE must be 0..4";
     else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 (fp64) or 1 (mixed)";
-    else if (opt->boundary < 0 || opt->boundary > 1) why = "boundary must be 0 (periodic) or 1 (walls)";
+    else if (opt->boundary < 0 || opt->boundary > 3)
+        why = "boundary must be 0 (periodic), 1 (slip walls), 2 (no-slip walls) or 3 (fixed ghost cells)";
     else if (!(opt->cfl > 0)) why = "CFL must be positive";
     else if (opt->geometry_ranks < 0 || (opt->geometry_ranks > 1 && opt->elem_num_x % opt->geometry_ranks))
         why = "ELEMNUMX must be divisible by the number of ranks of the reference grid";
@@ -183,6 +185,10 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     init.active = 1;
     if (bad(hipMemcpy(c->d_ctl, &init, sizeof init, hipMemcpyHostToDevice))) return WFV_ERR_HIP;
     if (bad(hipMemset(c->reg[0], 0, 4 * c->nc * sizeof(double)))) return WFV_ERR_HIP;
+    if (opt->boundary == 3) {
+        const size_t bytes = 4 * ring_size(c->nx, c->ny) * sizeof(double);
+        if (bad(hipMalloc(&c->d_ghost, bytes)) || bad(hipMemset(c->d_ghost, 0, bytes))) return WFV_ERR_HIP;
+    }
     *out = c;
     return WFV_OK;
 }
@@ -197,6 +203,7 @@ void wfv_destroy(wfv_ctx *c) {
     (void)hipFree(c->d_dyC);
     (void)hipFree(c->d_faces);
     (void)hipFree(c->d_theta);
+    (void)hipFree(c->d_ghost);
     (void)hipFree(c->d_ctl);
     (void)hipFree(c->d_dts);
     if (c->own_stream) {
@@ -359,6 +366,16 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
     return WFV_OK;
 }
 
+int wfv_set_ghost_cells(wfv_ctx *c, const double *Upad) {
+    if (!c || !Upad) return WFV_ERR_ARG;
+    if (!c->d_ghost) return fail(c, WFV_ERR_CONFIG, "ghost cells need boundary 3");
+    std::vector<double> ring(4 * ring_size(c->nx, c->ny));
+    pack_ghost_ring(c->nx, c->ny, Upad, ring.data());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_ghost, ring.data(), ring.size() * sizeof(double), hipMemcpyHostToDevice));
+    return WFV_OK;
+}
+
 int wfv_get_flags(wfv_ctx *c, unsigned *flags) {
     if (!c || !flags) return WFV_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -486,7 +503,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     for (int i = 0; i < n && why.empty(); ++i) {
         const wfv_instance &I = inst[i];
         if (I.nx < 3 || I.ny < 3) why = "instance grids need at least 3 cells per direction";
-        else if (I.boundary < 0 || I.boundary > 1) why = "instance boundary must be 0 or 1";
+        else if (I.boundary < 0 || I.boundary > 3) why = "instance boundary must be 0..3";
         else if (!(I.cs.xR > I.cs.xL) || !(I.cs.yR > I.cs.yL)) why = "empty instance domain";
         else if (opt->geometry_ranks > 1 && I.nx % opt->geometry_ranks) why = "nx not divisible by geometry_ranks";
     }
@@ -501,7 +518,13 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     b->hinst.resize(n);
     b->cell_off.resize(n);
     b->ty.resize(n);
-    size_t geo_total = 0;
+    size_t geo_total = 0, ghost_total = 0;
+    std::vector<size_t> ghost_off(n, 0);
+    for (int i = 0; i < n; ++i)
+        if (inst[i].boundary == 3) {
+            ghost_off[i] = ghost_total;
+            ghost_total += 4 * ring_size(inst[i].nx, inst[i].ny);
+        }
     std::vector<int> nxs(n), nys(n);
     for (int i = 0; i < n; ++i) {
         nxs[i] = inst[i].nx;
@@ -530,6 +553,9 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         bad(hipMalloc(&b->d_inst, n * sizeof(BatchInst))))
         return WFV_ERR_HIP;
     if (bad(hipMemset(b->reg[0], 0, 4 * b->total_cells * sizeof(double)))) return WFV_ERR_HIP;
+    if (ghost_total &&
+        (bad(hipMalloc(&b->ghost, ghost_total * sizeof(double))) || bad(hipMemset(b->ghost, 0, ghost_total * sizeof(double)))))
+        return WFV_ERR_HIP;
     std::vector<double> geo(geo_total);
     size_t go = 0;
     for (int i = 0; i < n; ++i) {
@@ -559,6 +585,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         I.nx = in.nx;
         I.ny = in.ny;
         I.bc = in.boundary;
+        I.ghost = in.boundary == 3 ? b->ghost + ghost_off[i] : nullptr;
     }
     if (bad(hipMemcpy(b->geo, geo.data(), geo_total * sizeof(double), hipMemcpyHostToDevice))) return WFV_ERR_HIP;
     std::vector<StepCtl> ctl(n);
@@ -582,6 +609,7 @@ void wfv_batch_destroy(wfv_batch *b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (auto &p : b->reg) (void)hipFree(p);
     (void)hipFree(b->geo);
+    (void)hipFree(b->ghost);
     (void)hipFree(b->d_ctl);
     (void)hipFree(b->d_inst);
     (void)hipFree(b->d_item_inst);
@@ -604,6 +632,17 @@ int wfv_batch_get_state(wfv_batch *b, int i, double *U) {
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     BHIPCHK(b, hipMemcpy(U, b->hinst[i].reg[0], 4 * (size_t)b->inst[i].nx * b->inst[i].ny * sizeof(double),
                          hipMemcpyDeviceToHost));
+    return WFV_OK;
+}
+
+int wfv_batch_set_ghost_cells(wfv_batch *b, int i, const double *Upad) {
+    if (!b || !Upad || i < 0 || i >= b->n) return WFV_ERR_ARG;
+    if (!b->hinst[i].ghost) return bfail(b, WFV_ERR_CONFIG, "ghost cells need boundary 3");
+    const int nx = b->inst[i].nx, ny = b->inst[i].ny;
+    std::vector<double> ring(4 * ring_size(nx, ny));
+    pack_ghost_ring(nx, ny, Upad, ring.data());
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    BHIPCHK(b, hipMemcpy((void *)b->hinst[i].ghost, ring.data(), ring.size() * sizeof(double), hipMemcpyHostToDevice));
     return WFV_OK;
 }
 

@@ -18,6 +18,7 @@ struct wfv_ctx {
     double *reg[3] = {nullptr, nullptr, nullptr};  // RK registers; reg[0] is the state
     double *d_dxC = nullptr, *d_dxW = nullptr, *d_dyC = nullptr;
     double *d_faces = nullptr, *d_theta = nullptr;
+    double *d_ghost = nullptr;        // fixed ghost ring (boundary 3)
     wfv::StepCtl *d_ctl = nullptr;
     double *d_dts = nullptr;
     int dts_cap = 0;
@@ -45,6 +46,7 @@ struct wfv_batch {
     int max_cells = 0;
     double *reg[3] = {nullptr, nullptr, nullptr};
     double *geo = nullptr;              // all instances' dxC, dxW, dyC
+    double *ghost = nullptr;            // fixed ghost rings of the boundary-3 instances
     wfv::StepCtl *d_ctl = nullptr;
     wfv::BatchInst *d_inst = nullptr;
     int *d_item_inst = nullptr;
@@ -65,6 +67,18 @@ struct StagePlan {
 int rk_plan(int rk, StagePlan *s);
 
 int fail(wfv_ctx *c, int code, const std::string &msg);
+
+// a padded state [4][ny+2G][nx+2G] -> its ghost ring, SoA [4][ring_size(nx, ny)]
+inline void pack_ghost_ring(int nx, int ny, const double *Upad, double *ring) {
+    const int G = WFV_GHOST, px = nx + 2 * G, py = ny + 2 * G;
+    const size_t rs = ring_size(nx, ny), pc = (size_t)px * py;
+    for (int v = 0; v < 4; ++v)
+        for (int j = -G; j < ny + G; ++j)
+            for (int i = -G; i < nx + G; ++i) {
+                if (i >= 0 && i < nx && j >= 0 && j < ny) continue;
+                ring[v * rs + ghost_offset(nx, ny, i, j)] = Upad[v * pc + (size_t)(j + G) * px + (i + G)];
+            }
+}
 int bfail(wfv_batch *b, int code, const std::string &msg);
 
 // stage arguments of a context without registers, RK coefficients or dt

@@ -11,6 +11,28 @@ namespace wfv {
 
 constexpr int WAVE_LANES = 64;
 
+// cells of a fixed-ghost ring (boundary 3) per variable, WFV_GHOST layers around nx x ny
+__host__ __device__ inline size_t ring_size(int nx, int ny) {
+    return 2 * (size_t)WFV_GHOST * (nx + 2 * WFV_GHOST) + 2 * (size_t)WFV_GHOST * ny;
+}
+
+// Fixed ghost cells (boundary 3): a ring of WFV_GHOST layers around the nx x ny interior,
+// SoA over the 4 variables with ring_size(nx, ny) cells each: the bottom rows (-G..-1) and
+// top rows (ny..ny+G-1) over columns -G..nx+G-1, then the left and right columns over rows
+// 0..ny-1.  Columns beyond the ring (lanes past the last strip's halo) clamp to its edge.
+__host__ __device__ inline size_t ghost_offset(int nx, int ny, int i, int j) {
+    constexpr int G = WFV_GHOST;
+    const int px = nx + 2 * G;
+    i = i < -G ? -G : (i >= nx + G ? nx + G - 1 : i);
+    j = j < -G ? -G : (j >= ny + G ? ny + G - 1 : j);
+    if (j < 0) return (size_t)(j + G) * px + (i + G);
+    if (j >= ny) return (size_t)G * px + (size_t)(j - ny) * px + (i + G);
+    const size_t sides = 2 * (size_t)G * px;
+    if (i < 0) return sides + (size_t)j * G + (i + G);
+    if (i >= nx) return sides + (size_t)ny * G + (size_t)j * G + (i - nx);
+    return 0;  // interior: not a ghost cell
+}
+
 // Stencil of a scheme: radius R = 2 for the 5-point WENO-JS/Z, 3 for the 7-point WENO-EXP
 // (SCHEME 2) and primitive3 (RECONSTRUCTIONTYPE 4).  A strip keeps R+1 halo columns on each
 // side (R for the stencil, one for the neighbouring strips' edge cells), so a wavefront owns
@@ -38,6 +60,7 @@ struct StageArgs {
     Phys P;
     double dX, dY, cfl, gravity;
     double dx4;           // pow(dX, 4): WENO-EXP's smoothness threshold, rounded as the host's libm
+    const double *ghost;  // fixed ghost ring (boundary 3), SoA [4][ring_size]
     int nx, ny, bc, flux, bp;
     int ty;               // rows per strip
     int strips_x, strips_y;
@@ -79,6 +102,7 @@ struct BatchInst {
     Phys P;
     double dX, dY, cfl, gravity;
     double dx4;                  // pow(dX, 4) (WENO-EXP)
+    const double *ghost;         // fixed ghost ring (boundary 3) or nullptr
     int nx, ny, bc, ty;
     int strips_x, strips_y;
     int item_begin;              // first item of this instance in the flattened list

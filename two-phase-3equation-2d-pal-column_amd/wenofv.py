@@ -18,6 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("WENOFV_LIB", os.path.join(HERE, "libwenofv.so"))
 
 OK, ERR_CONFIG, ERR_HIP, ERR_NONFINITE_DT, ERR_ARG = 0, 1, 2, 3, 4
+GHOST = 4  # WFV_GHOST: ghost layers of a fixed-ghost boundary (bc 3)
+# boundary kinds: 0 periodic, 1 slip walls (NEUMANN, FREESLIP), 2 no-slip walls (NoSlip),
+# 3 fixed ghost cells (NEUMANN2; set them with set_ghost_cells)
 FLAG_NAN_FLUX, FLAG_BAD_THETA, FLAG_NAN_DT = 1, 2, 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -98,6 +101,7 @@ def lib():
     L.wfv_set_stream.argtypes = [vp, vp]
     L.wfv_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp]
     L.wfv_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Tuning)]
+    L.wfv_set_ghost_cells.argtypes = [vp, _dp]
     L.wfv_set_rows_per_strip.argtypes = [vp, ctypes.c_int]
     L.wfv_get_rows_per_strip.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.wfv_version.restype = ctypes.c_char_p
@@ -150,6 +154,12 @@ class Solver:
         U = np.empty((4, self.ny, self.nx))
         self._check(lib().wfv_get_state(self._h, U.ctypes.data))
         return U
+
+    def set_ghost_cells(self, Upad):
+        """Ghost cells of a boundary-3 solver: a padded state (4, ny + 2*GHOST, nx + 2*GHOST)."""
+        Upad = np.ascontiguousarray(Upad, dtype=np.float64)
+        assert Upad.shape == (4, self.ny + 2 * GHOST, self.nx + 2 * GHOST)
+        self._check(lib().wfv_set_ghost_cells(self._h, _p(Upad)))
 
     def set_state_device(self, ptr):
         self._check(lib().wfv_set_state_device(self._h, ctypes.c_void_p(ptr)))
@@ -259,6 +269,7 @@ def _batch_lib():
     L.wfv_batch_set_schedule.argtypes = [vp, ctypes.c_int]
     L.wfv_batch_get_schedule.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.wfv_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(Tuning)]
+    L.wfv_batch_set_ghost_cells.argtypes = [vp, ctypes.c_int, _dp]
     L._batch_ready = True
     return L
 
@@ -306,6 +317,12 @@ class Batch:
         U = np.ascontiguousarray(U, dtype=np.float64)
         assert U.shape == (4,) + self.shapes[i]
         self._check(_batch_lib().wfv_batch_set_state(self._h, i, U.ctypes.data))
+
+    def set_ghost_cells(self, i, Upad):
+        ny, nx = self.shapes[i]
+        Upad = np.ascontiguousarray(Upad, dtype=np.float64)
+        assert Upad.shape == (4, ny + 2 * GHOST, nx + 2 * GHOST)
+        self._check(_batch_lib().wfv_batch_set_ghost_cells(self._h, i, _p(Upad)))
 
     def get_state(self, i):
         U = np.empty((4,) + self.shapes[i])
@@ -386,6 +403,16 @@ def reference_case(testcase):
     kw = dict(domain=(c.xL, c.xR, c.yL, c.yR), rho10=c.rho10, rho20=c.rho20, sound1=c.sound1, sound2=c.sound2,
               preb=c.preb, gravity=c.gravity, bc=bc.value)
     return kw, t.value
+
+
+def reference_initial_state_padded(testcase, nx, ny, geometry_ranks=4):
+    """The same on the grid padded by GHOST cells per side (ghost cells of bc 3)."""
+    L = lib()
+    L.wfv_reference_initial_state_padded.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+    U = np.empty((4, ny + 2 * GHOST, nx + 2 * GHOST))
+    if L.wfv_reference_initial_state_padded(testcase, nx, ny, geometry_ranks, _p(U)) != OK:
+        raise WenofvError("bad arguments")
+    return U
 
 
 def reference_initial_state(testcase, nx, ny, geometry_ranks=4):
