@@ -107,6 +107,16 @@ int wfv_advance(wfv_ctx *ctx, double t_start, double t_end, int fixed_steps, int
    src/WENOFV.cpp:2287-2292). */
 int wfv_set_ghost_cells(wfv_ctx *ctx, const double *Upad);
 
+/* Limiter flags of the reference's outputs (m_totalTheta, src/WENOFV.cpp:1906-1909): with
+   recording on, every stage stores per cell 1.0 where the limiter's theta < 1-1e-8, else
+   0.0; wfv_get_limiter_flags returns those of the last stage, [ny][nx]. */
+int wfv_record_limiter_flags(wfv_ctx *ctx, int on);
+int wfv_get_limiter_flags(wfv_ctx *ctx, double *flags);
+
+/* The same time loop toward t_end, stopping early after max_steps steps (>= 1): lets a
+   caller write the reference's step-count outputs (src/WENOFV.cpp:2483-2520) between runs. */
+int wfv_advance_steps(wfv_ctx *ctx, double t_start, double t_end, int max_steps, int *nsteps, double *t_out);
+
 /* Error bits raised by the kernels since the last call (WFV_FLAG_*); clears them. */
 int wfv_get_flags(wfv_ctx *ctx, unsigned *flags);
 int wfv_synchronize(wfv_ctx *ctx);

@@ -196,6 +196,23 @@ int main(int argc, char **argv) {
             write_file((out + "_dts.bin").c_str(), dts);
             std::printf("steps %d t %.17g loop_seconds %.6f\n", count, now, secs);
         }
+    } else if (mode == "outputs") {
+        // the reference's run() loop (src/WENOFV.cpp:305-373) with its output calls, one output
+        // time t_end: the reference writes its own .plt / .csv files under ./output/
+        double now = 0.0;
+        int count = 0;
+        solver.outputResults(count, now, "initial");
+        while (std::abs(now - tend) > 1e-10) {
+            double timestep = solver.calculateDeltaT(now);
+            timestep = std::min(timestep, tend - now);
+            solver.performTimeIntegration(timestep);
+            now += timestep;
+            count++;
+            solver.outputResults(count, now);
+        }
+        solver.outputResults(count, now, "intermediate");
+        solver.outputResults(count, now, "final");
+        if (solver.m_rank == 0) std::printf("steps %d t %.17g dir %s\n", count, now, solver.m_outputDir.c_str());
     } else if (mode == "theta") {
         // file: ncase records of [U(4) | faces(64) in [side][v][g]]; rank 0 evaluates them
         // one by one in interior cell (3,3).

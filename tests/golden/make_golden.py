@@ -399,6 +399,34 @@ def make_runs_exp():
     print("WENO-EXP / primitive3 run fixtures written")
 
 
+OUTPUT_RUNS = [
+    # name, testcase, nx, ny, rk, cfl, scheme, bp, flux, recon, t_end
+    ("smooth16_js_llf_cons", 0, 16, 16, 2, 0.4, 0, 0, 0, 0, 0.01),
+    ("smooth16_z_hllc_char_bp", 0, 16, 16, 2, 0.4, 1, 1, 1, 1, 0.01),
+    ("vortex16_z_llf_cons_bp", 1, 16, 16, 2, 0.4, 1, 1, 0, 0, 0.002),
+    ("sharp16_z_hllc_cons_bp", 2, 16, 16, 2, 0.4, 1, 1, 1, 0, 0.04),
+]
+
+
+def make_outputs():
+    """The files the reference's own run loop writes (CWENOFV::outputResults and the writers
+    it calls) for short runs: Tecplot averages, accuracy CSV/averages, SHARP's time series.
+    Only the .plt/.csv outputs are kept (not the source backup the reference also writes)."""
+    out = {}
+    for name, tc, nx, ny, rk, cfl, scheme, bp, flux, recon, tend in OUTPUT_RUNS:
+        with tempfile.TemporaryDirectory() as tmp:
+            _, so = run_ref(["outputs", tc, nx, ny, rk, cfl, scheme, bp, flux, recon, tend], tmp)
+            (d,) = [os.path.join(tmp, "output", x) for x in os.listdir(os.path.join(tmp, "output"))]
+            files = sorted(f for f in os.listdir(d) if f.endswith((".plt", ".csv")))
+            out[name + "__opts"] = np.array([tc, nx, ny, rk, cfl, scheme, bp, flux, recon, tend])
+            out[name + "__files"] = np.array(files)
+            for f in files:
+                out[name + "__" + f] = np.frombuffer(open(os.path.join(d, f), "rb").read(), dtype=np.uint8)
+            print(name, len(files), "files;", so.strip().splitlines()[-1])
+    np.savez_compressed(os.path.join(HERE, "reference_outputs.npz"), **out)
+    print("output fixtures written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", default=None)
@@ -411,7 +439,8 @@ def main():
             # separate generators, so the fixtures above stay unchanged
             "pointexp": lambda: make_point_exp(np.random.default_rng(7007)),
             "rhsexp": lambda: make_rhs_exp(np.random.default_rng(7008)),
-            "runsexp": make_runs_exp}
+            "runsexp": make_runs_exp,
+            "outputs": make_outputs}
     for k, f in jobs.items():
         if a.only is None or k in a.only:
             f()

@@ -30,6 +30,7 @@ UNITS = [
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),
     ("wenofv_tune.cpp", ["-ffp-contract=off"]),
     ("WENOFV.cpp", ["-ffp-contract=off"]),
+    ("WENOFV_output.cpp", ["-ffp-contract=off"]),
 ]
 CLI = os.path.join(HERE, "wenofv_run")
 

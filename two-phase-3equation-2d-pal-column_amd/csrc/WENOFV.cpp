@@ -49,6 +49,8 @@ void CWENOFV::initializeSolver() {
     m_reconstructionType = (RECONSTRUCTIONTYPE)stoi_key("RECONSTRUCTIONTYPE");
     equation->setEquationParameters(m_testcase);
     m_timeMoments = equation->outputTimes;
+    // ENDTIME (no reference counterpart): replace the case's output times by {0, ENDTIME}
+    if (option.count("ENDTIME")) m_timeMoments = {0.0, std::stod(option["ENDTIME"])};
     m_scheme = (SCHEMETYPE)stoi_key("SCHEME");
     m_pplimiter = stoi_key("PPlimiter") != 0;
     auto cfl = option.find("CFL");
@@ -90,9 +92,11 @@ void CWENOFV::initializeSolver() {
     ctx_ = nullptr;
     const int rc = wfv_create(&o, &c, &ctx_);
     if (rc != WFV_OK) throw std::runtime_error(std::string("wfv_create: ") + wfv_last_error(nullptr));
-    m_outputDir = "./output/" + TESTCASETYPE_STRINGS[m_testcase] + "_" + SCHEMETYPE_STRINGS[m_scheme] + "_" +
+    // ./output/<timestamp>_<case>_<scheme>_<reconstruction>_<flux>_<nx>x<ny>/ (src/WENOFV.cpp:100-105)
+    m_outputDir = "./output/" + timestamp() + "_" + TESTCASETYPE_STRINGS[m_testcase] + "_" + SCHEMETYPE_STRINGS[m_scheme] + "_" +
                   RECONSTRUCTIONTYPE_STRINGS[m_reconstructionType] + "_" + RIEMANNFLUXTYPE_STRINGS[m_riemannFluxType] +
                   "_" + std::to_string(m_worldElemNumX) + "x" + std::to_string(m_worldElemNumY) + "/";
+    if (m_pplimiter) check(wfv_record_limiter_flags(ctx_, 1), "record limiter flags");
     initializeAve();
 }
 
@@ -206,42 +210,44 @@ void CWENOFV::readState(const std::string &path) {
 }
 
 void CWENOFV::run() {
+    // CWENOFV::run (src/WENOFV.cpp:281-374): the loop over the output times with the
+    // reference's outputs.  The steps between two outputs run on the device
+    // (wfv_advance_steps: clipped CFL steps, up to the next step count that writes a file).
     const auto t0 = std::chrono::steady_clock::now();
     initializeSolver();
     double now = 0;
     int count = 0;
-    if (m_testcase == SMOOTH || m_testcase == VORTEX) outputAccuracy("initial", now);
+    outputResults(count, now, "initial");
     const auto t1 = std::chrono::steady_clock::now();
-    // loop over the output times (src/WENOFV.cpp:318-364); the steps between two output
-    // times run on the device (wfv_advance replays the clipped CFL steps from a graph)
+    double loop = 0;
     for (size_t k = 1; k < m_timeMoments.size(); ++k) {
-        int n = 0;
-        double t = now;
-        check(wfv_advance(ctx_, now, m_timeMoments[k], 0, &n, &t, nullptr, 0), "run");
-        now = t;
-        count += n;
+        const double target = m_timeMoments[k];
+        while (std::abs(now - target) > 1e-10) {
+            const auto s0 = std::chrono::steady_clock::now();
+            int n = 0;
+            double t = now;
+            check(wfv_advance_steps(ctx_, now, target, nextOutputCount(count) - count, &n, &t), "run");
+            loop += std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
+            if (n == 0) break;
+            now = t;
+            count += n;
+            outputResults(count, now);
+        }
+        outputResults(count, now, "intermediate");
         std::cout << "t = " << now << "  steps " << count << std::endl;
     }
-    const auto t2 = std::chrono::steady_clock::now();
     unsigned flags = 0;
     check(wfv_get_flags(ctx_, &flags), "flags");
     if (flags & WFV_FLAG_NAN_FLUX) std::cerr << "Riemann flux is nan." << std::endl;
     if (flags & WFV_FLAG_BAD_THETA) std::cerr << "Error: illegal theta in PP limiting..." << std::endl;
     std::cout << "\nTotal iterations completed. Total iterations: " << count << std::endl;
-    if (m_testcase == SMOOTH || m_testcase == VORTEX || m_testcase == SHARP) {
-        const auto e = outputAccuracy("final", now);
-        std::cout << std::setprecision(15) << std::scientific << "accuracy (Linf, L1, L2): " << e[0] << " " << e[1]
-                  << " " << e[2] << std::endl;
-    } else {
-        getWorldUh();
-    }
-    std::filesystem::create_directories(m_outputDir);
+    outputResults(count, now, "final");
     writeState(m_outputDir + "final_state.bin");
-    const double loop = std::chrono::duration<double>(t2 - t1).count();
     const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const double cs = 3.0 * count * m_worldElemNumX * m_worldElemNumY;
-    std::cout << std::defaultfloat << "time loop " << loop << " s (" << cs / loop / 1e6
+    std::cout << std::defaultfloat << "time loop " << loop << " s (" << (loop > 0 ? cs / loop / 1e6 : 0.0)
               << " M cell-stage/s for RK3), total " << total << " s" << std::endl;
+    (void)t1;
 }
 
 extern "C" int wfv_reference_case(int testcase, wfv_case *cs, int *boundary, double *t_end) {

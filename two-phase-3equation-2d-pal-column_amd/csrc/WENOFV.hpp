@@ -83,6 +83,19 @@ public:
     // CWENOFV::outputAccuracy (src/WENOFV.cpp:3152-3251): L-inf, L1, L2 of theVarUh against the
     // exact cell averages; returns them and appends the reference's CSV line
     std::vector<double> outputAccuracy(const std::string &prefix, double now);
+    // CWENOFV::outputResults and the output files it writes (src/WENOFV.cpp:2447-3391): the
+    // reference's cadence (initial, step 1, every 200 / 1000 steps, output times, final)
+    void outputResults(int count, double now, const std::string &specialCase = "");
+    void outputAve(const std::string &prefix);
+    void outputSliceAve(const std::string &prefix);
+    void outputAccuracyAve(const std::string &prefix, double now);
+    void outputUPoscillation(const std::string &prefix);
+    void outputUPoscillationByTime(double now);
+    void outputThincknessByTime(double now);
+    void outputSymmetryErr(double now);
+    void outputPressureByTime(double now);
+    void outputSpecialTestCasebyTime(double now);
+    void outputSpecialTestCasebyCell(const std::string &prefix);
     // state dump / restore (raw FP64 SoA with a small header)
     void writeState(const std::string &path) const;
     void readState(const std::string &path);
@@ -91,6 +104,13 @@ public:
 
 private:
     wfv_ctx *ctx_ = nullptr;
+    std::vector<double> m_flags;  // limiter flags of the last stage, [ny][nx] (theta column)
+    static std::string timestamp();
+    double xCenter(int wi) const;  // world grid cell centres, ghost-offset index
+    double yCenter(int wj) const;
+    void snapshot();                                  // device state + flags -> host
+    void cell(int ix, int iy, double U[4]) const;     // interior cell of the snapshot
+    int nextOutputCount(int count) const;             // next step count with an output
     void check(int rc, const char *what) const;
     void rk_checked(RKMETHOD m, double dt);
     int stoi_key(const char *key);

@@ -682,6 +682,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     th[k] = scMin(fabs((uave - eps) / (uave - mloc)), 1.0);
                 }
                 theta = scMin(th[0], th[1]);
+                // the limiter flag of the reference's outputs (src/WENOFV.cpp:1906-1909)
+                if (A.theta_flag && owned && r >= y0 && r < y1)
+                    A.theta_flag[(size_t)r * nx + xc] = theta < 1 - 1e-8 ? 1.0 : 0.0;
                 if (theta >= 0.0 && theta <= 1.0) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
@@ -890,6 +893,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.dtmin = B.want_dtmin ? &ctl->dtmin_bits : nullptr;
     A.faces_out = nullptr;
     A.theta_out = nullptr;
+    A.theta_flag = nullptr;
     stage_body<PR, RECON, SCHEME, FLUX>(A, item - I.item_begin, L);
 }
 

@@ -51,6 +51,7 @@ StageArgs base_args(wfv_ctx *c) {
     A.ny = c->ny;
     A.bc = c->opt.boundary;
     A.ghost = c->d_ghost;
+    A.theta_flag = c->d_theta_flag;
     A.flux = c->opt.riemann_flux;
     A.bp = c->opt.pp_limiter;
     A.ty = c->ty;
@@ -204,6 +205,7 @@ void wfv_destroy(wfv_ctx *c) {
     (void)hipFree(c->d_faces);
     (void)hipFree(c->d_theta);
     (void)hipFree(c->d_ghost);
+    (void)hipFree(c->d_theta_flag);
     (void)hipFree(c->d_ctl);
     (void)hipFree(c->d_dts);
     if (c->own_stream) {
@@ -282,6 +284,7 @@ int wfv_assemble_rhs(wfv_ctx *c, double *rhs, double *faces, double *theta) {
     A.b = 0.0;
     A.c = 1.0;
     A.dt = 1.0;
+    A.theta_flag = nullptr;
     A.faces_out = (faces || theta) ? c->d_faces : nullptr;
     A.theta_out = (faces || theta) ? c->d_theta : nullptr;
     HIPCHK(c, launch_stage(A, c->opt.precision, c->opt.reconstruction, c->opt.scheme, c->stream));
@@ -292,8 +295,14 @@ int wfv_assemble_rhs(wfv_ctx *c, double *rhs, double *faces, double *theta) {
     return WFV_OK;
 }
 
-int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *nsteps, double *t_out,
-                double *dts, int max_dts) {
+}  // extern "C"
+
+namespace {
+
+// the device-resident time loop: fixed unclipped steps (fixed_steps > 0), or toward t_end with
+// the last step clipped, stopping early after max_count steps when max_count > 0
+int advance_impl(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int max_count, int *nsteps, double *t_out,
+                 double *dts, int max_dts) {
     if (!c) return WFV_ERR_ARG;
     if (dts && max_dts > c->dts_cap) {
         (void)hipFree(c->d_dts);
@@ -312,6 +321,7 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
     h.active = 1;
     h.dts = dts ? c->d_dts : nullptr;
     h.max_record = dts ? max_dts : 0;
+    h.max_count = fixed_steps > 0 ? 0 : max_count;
     HIPCHK(c, hipMemcpy(c->d_ctl, &h, sizeof h, hipMemcpyHostToDevice));
     // CFL minimum of the initial state
     DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags, &c->d_ctl->dtmin_bits};
@@ -337,8 +347,11 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
         if (e != hipSuccess) return fail(c, WFV_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
         c->graph_steps = GRAPH_STEPS;
     }
-    if (fixed_steps > 0) {
-        const int full = fixed_steps / GRAPH_STEPS, rest = fixed_steps % GRAPH_STEPS;
+    // a short capped run is enqueued step by step rather than as graph replays of no-op steps
+    const int capped = (fixed_steps <= 0 && max_count > 0 && max_count < GRAPH_STEPS) ? max_count : 0;
+    if (fixed_steps > 0 || capped) {
+        const int nfix = fixed_steps > 0 ? fixed_steps : capped;
+        const int full = nfix / GRAPH_STEPS, rest = nfix % GRAPH_STEPS;
         for (int k = 0; k < full; ++k) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
         for (int k = 0; k < rest; ++k) {
             HIPCHK(c, launch_step_control(c->d_ctl, c->stream));
@@ -366,6 +379,20 @@ int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *
     return WFV_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int wfv_advance(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int *nsteps, double *t_out,
+                double *dts, int max_dts) {
+    return advance_impl(c, t_start, t_end, fixed_steps, 0, nsteps, t_out, dts, max_dts);
+}
+
+int wfv_advance_steps(wfv_ctx *c, double t_start, double t_end, int max_steps, int *nsteps, double *t_out) {
+    if (max_steps < 1) return WFV_ERR_ARG;
+    return advance_impl(c, t_start, t_end, 0, max_steps, nsteps, t_out, nullptr, 0);
+}
+
 int wfv_set_ghost_cells(wfv_ctx *c, const double *Upad) {
     if (!c || !Upad) return WFV_ERR_ARG;
     if (!c->d_ghost) return fail(c, WFV_ERR_CONFIG, "ghost cells need boundary 3");
@@ -373,6 +400,31 @@ int wfv_set_ghost_cells(wfv_ctx *c, const double *Upad) {
     pack_ghost_ring(c->nx, c->ny, Upad, ring.data());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->d_ghost, ring.data(), ring.size() * sizeof(double), hipMemcpyHostToDevice));
+    return WFV_OK;
+}
+
+int wfv_record_limiter_flags(wfv_ctx *c, int on) {
+    if (!c) return WFV_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (on && !c->d_theta_flag) {
+        HIPCHK(c, hipMalloc(&c->d_theta_flag, c->nc * sizeof(double)));
+        HIPCHK(c, hipMemset(c->d_theta_flag, 0, c->nc * sizeof(double)));
+    } else if (!on && c->d_theta_flag) {
+        (void)hipFree(c->d_theta_flag);
+        c->d_theta_flag = nullptr;
+    }
+    if (c->graph) {  // the captured steps carry the flag array
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return WFV_OK;
+}
+
+int wfv_get_limiter_flags(wfv_ctx *c, double *flags) {
+    if (!c || !flags) return WFV_ERR_ARG;
+    if (!c->d_theta_flag) return fail(c, WFV_ERR_CONFIG, "limiter flags are not recorded");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(flags, c->d_theta_flag, c->nc * sizeof(double), hipMemcpyDeviceToHost));
     return WFV_OK;
 }
 

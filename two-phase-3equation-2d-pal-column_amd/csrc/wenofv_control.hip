@@ -50,14 +50,15 @@ __global__ __launch_bounds__(DT_THREADS) void dt_kernel(DtArgs A) {
 
 // Device-side step control of the reference time loop (src/WENOFV.cpp:318-332):
 // dt = min(timestep, T - now) with timestep = min(1.0, CFL minimum); the loop stops when
-// |now - T| <= 1e-10, or after a fixed number of steps.  It writes the dt the following
+// |now - T| <= 1e-10 (or after max_count steps, where the caller writes output), or after a
+// fixed number of unclipped steps.  It writes the dt the following
 // stages read, advances `now`, and re-arms the CFL minimum for the last stage.
 __global__ void step_control_kernel(StepCtl *S) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (S->active == 0) return;
     bool go;
     if (S->fixed_steps > 0) go = S->count < S->fixed_steps;
-    else go = fabs(S->now - S->t_end) > 1e-10;
+    else go = fabs(S->now - S->t_end) > 1e-10 && (S->max_count <= 0 || S->count < S->max_count);
     if (!go || (S->flags & WFV_FLAG_NAN_DT)) {
         S->active = 0;
         return;

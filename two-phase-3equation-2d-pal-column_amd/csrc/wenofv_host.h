@@ -19,6 +19,7 @@ struct wfv_ctx {
     double *d_dxC = nullptr, *d_dxW = nullptr, *d_dyC = nullptr;
     double *d_faces = nullptr, *d_theta = nullptr;
     double *d_ghost = nullptr;        // fixed ghost ring (boundary 3)
+    double *d_theta_flag = nullptr;   // limiter flags of the last stage (when recorded)
     wfv::StepCtl *d_ctl = nullptr;
     double *d_dts = nullptr;
     int dts_cap = 0;

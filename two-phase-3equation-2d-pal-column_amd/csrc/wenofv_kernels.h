@@ -68,6 +68,7 @@ struct StageArgs {
     unsigned long long *dtmin;       // if set: CFL minimum of the output state (bit pattern)
     double *faces_out;               // parity hook: limited face values [cell][side][v][g]
     double *theta_out;               // parity hook: limiter theta per cell
+    double *theta_flag;              // if set: theta < 1-1e-8 per cell, overwritten every stage
 };
 
 struct DtArgs {
@@ -91,7 +92,7 @@ struct StepCtl {
     unsigned flags;                 // error bits
     double *dts;                    // optional record of every dt (device memory)
     int max_record;
-    int pad;
+    int max_count;                  // time-driven loop: stop after this many steps (0: no cap)
 };
 
 // one instance of a batch (device memory)
