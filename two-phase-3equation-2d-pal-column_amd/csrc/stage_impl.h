@@ -178,7 +178,7 @@ struct FaceSide<PrecMixed> {
     }
     WFV_DEV void limit(int v, int g, double theta, double ub) {  // theta*(value - ub) + ub
         if (v < 2) r[v & 1][g] = theta * (r[v & 1][g] - ub) + ub;
-        else m[v & 1][g] = float(theta * double(m[v & 1][g]));
+        else m[v & 1][g] *= float(theta);  // a deviation from ub itself
     }
     WFV_DEV void rotate() {
 #pragma unroll
@@ -290,6 +290,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr int R = ST::R, NW = ST::NW, HALO = ST::HALO, SW = ST::SW;
     constexpr bool EX = PR::exact;
     constexpr bool DEV = !EX && R == 2;
+    // Loops over the 4 variables: unrolled in the mixed policy (no register rotation, no
+    // spills, about 14 % fewer instructions per cell), rolled in the all-FP64 policy, whose
+    // unrolled code would outgrow the 64 KiB instruction cache.
+    constexpr int VU = EX ? 1 : 4;
     const int lane = threadIdx.x & (LANES - 1);
 
     if (strip >= A.strips_x * A.strips_y) return;
@@ -447,7 +451,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int t = 0; t < 5; ++t)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) sw[t][v] = sv(t, v);
-#pragma unroll 1
+#pragma unroll VU
                 for (int v = 0; v < 4; ++v) {  // variable v sits in slot 0
                     W s5[5] = {sw[0][0], sw[1][0], sw[2][0], sw[3][0], sw[4][0]};
                     W o[4];
@@ -587,7 +591,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     }
             } else {
                 double bc[4] = {Bc[0], Bc[1], Bc[2], Bc[3]};
-#pragma unroll 1
+#pragma unroll VU
                 for (int v = 0; v < 4; ++v) {  // variable v's base sits in bc[0]
 #pragma unroll
                     for (int gy = 0; gy < 4; ++gy) {
@@ -672,20 +676,26 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     if constexpr (EX) rem /= omegaR;
                     else rem *= 1.0 / omegaR;
                     double mloc = rem;
+                    // mixed: the hardware minimum (it differs from sc_math::Min only for NaN
+                    // operands, which the flux check reports) and a reciprocal
+                    auto mn = [](double a, double b) { return EX ? scMin(a, b) : fmin(a, b); };
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
-                        mloc = scMin(mloc, fs.get(k, g, uave));
-                        mloc = scMin(mloc, fn.get(k, g, uave));
-                        mloc = scMin(mloc, fw.get(k, g, uave));
-                        mloc = scMin(mloc, fe.get(k, g, uave));
+                        mloc = mn(mloc, fs.get(k, g, uave));
+                        mloc = mn(mloc, fn.get(k, g, uave));
+                        mloc = mn(mloc, fw.get(k, g, uave));
+                        mloc = mn(mloc, fe.get(k, g, uave));
                     }
-                    th[k] = scMin(fabs((uave - eps) / (uave - mloc)), 1.0);
+                    if constexpr (EX) th[k] = scMin(fabs((uave - eps) / (uave - mloc)), 1.0);
+                    else th[k] = scMin(fabs((uave - eps) * rcp_nr(uave - mloc)), 1.0);
                 }
                 theta = scMin(th[0], th[1]);
                 // the limiter flag of the reference's outputs (src/WENOFV.cpp:1906-1909)
                 if (A.theta_flag && owned && r >= y0 && r < y1)
                     A.theta_flag[(size_t)r * nx + xc] = theta < 1 - 1e-8 ? 1.0 : 0.0;
-                if (theta >= 0.0 && theta <= 1.0) {
+                // mixed: theta = 1 leaves the values as they are, so a wavefront with no
+                // limited cell skips the loop
+                if (theta >= 0.0 && (EX ? theta <= 1.0 : theta < 1.0)) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const double ub = Uc[v];
@@ -697,7 +707,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                             fs.limit(v, g, theta, ub);
                         }
                     }
-                } else if (owned && r >= y0 && r < y1) {
+                } else if (!(theta >= 0.0 && theta <= 1.0) && owned && r >= y0 && r < y1) {
                     err |= WFV_FLAG_BAD_THETA;
                 }
             }

@@ -305,10 +305,13 @@ WFV_DEV void beta3(T a, T b, T c, T d, T e, T &b0, T &b1, T &b2) {
     t = c - T(2) * d + e;        b2 = b2 + T(13.0 / 12.0) * (t * t);
 }
 
+// Double (both policies' FP64 reconstructions): the reference's arithmetic, operation by
+// operation (include/Utility.hpp:5-95, 193-290).
 template <class T, int SCHEME>
 struct Weno5 {
+    static_assert(sizeof(T) == 8, "the float policy has its own specialisation below");
     T s0, s1, s2, s3, s4;
-    T k0, k1, k2;  // JS: (eps+beta_k)^2, or its reciprocal in float; Z: tau/(beta_k+eps)
+    T k0, k1, k2;  // JS: (eps+beta_k)^2; Z: tau/(beta_k+eps)
 
     WFV_DEV Weno5(T a, T b, T c, T d, T e) : s0(a), s1(b), s2(c), s3(d), s4(e) {
         T b0, b1, b2;
@@ -319,38 +322,20 @@ struct Weno5 {
             k0 = (eps + b0) * (eps + b0);
             k1 = (eps + b1) * (eps + b1);
             k2 = (eps + b2) * (eps + b2);
-            if constexpr (sizeof(T) == 4) {
-                k0 = __builtin_amdgcn_rcpf(k0);
-                k1 = __builtin_amdgcn_rcpf(k1);
-                k2 = __builtin_amdgcn_rcpf(k2);
-            }
         } else {
             // WENO5Zthreconstruction (include/Utility.hpp:261-280): beta of the min-max
             // normalised stencil, tau = |beta0 - beta2|, eps = 1e-10
             const T mx = wMax(wMax(wMax(a, b), wMax(e, d)), c);
             const T mn = wMin(wMin(wMin(a, b), wMin(e, d)), c);
             const T den = mx - mn + T(1e-20);
-            if constexpr (sizeof(T) == 8) {
-                const T a_ = (a - mn) / den, b_ = (b - mn) / den, c_ = (c - mn) / den;
-                const T d_ = (d - mn) / den, e_ = (e - mn) / den;
-                beta3(a_, b_, c_, d_, e_, b0, b1, b2);
-                const T eps = T(1.0e-10);
-                const T tau = fabs(b0 - b2);
-                k0 = dv(tau, b0 + eps);
-                k1 = dv(tau, b1 + eps);
-                k2 = dv(tau, b2 + eps);
-            } else {
-                // The indicators are shift-invariant quadratic forms, so those of the
-                // normalised stencil are beta/den^2 and tau/(beta_k + eps) on it equals
-                // tau_raw/(beta_k,raw + eps den^2) on the raw one.  The floor keeps a
-                // constant stencil at k = 0 where eps den^2 underflows.
-                beta3(a, b, c, d, e, b0, b1, b2);
-                const T e2 = wMax(T(1.0e-10) * den * den, T(1.17549435e-38f));
-                const T tau = fabs(b0 - b2);
-                k0 = tau * __builtin_amdgcn_rcpf(b0 + e2);
-                k1 = tau * __builtin_amdgcn_rcpf(b1 + e2);
-                k2 = tau * __builtin_amdgcn_rcpf(b2 + e2);
-            }
+            const T a_ = (a - mn) / den, b_ = (b - mn) / den, c_ = (c - mn) / den;
+            const T d_ = (d - mn) / den, e_ = (e - mn) / den;
+            beta3(a_, b_, c_, d_, e_, b0, b1, b2);
+            const T eps = T(1.0e-10);
+            const T tau = fabs(b0 - b2);
+            k0 = tau / (b0 + eps);
+            k1 = tau / (b1 + eps);
+            k2 = tau / (b2 + eps);
         }
     }
 
@@ -361,27 +346,72 @@ struct Weno5 {
         const T h2 = T(WENO_H[GP][2][0]) * s2 + T(WENO_H[GP][2][1]) * s3 + T(WENO_H[GP][2][2]) * s4;
         T w0, w1, w2;
         if constexpr (SCHEME == 0) {
-            if constexpr (sizeof(T) == 8) {
-                w0 = T(WENO_D[GP][0]) / k0;
-                w1 = T(WENO_D[GP][1]) / k1;
-                w2 = T(WENO_D[GP][2]) / k2;
-            } else {
-                w0 = T(WENO_D[GP][0]) * k0;
-                w1 = T(WENO_D[GP][1]) * k1;
-                w2 = T(WENO_D[GP][2]) * k2;
-            }
+            w0 = T(WENO_D[GP][0]) / k0;
+            w1 = T(WENO_D[GP][1]) / k1;
+            w2 = T(WENO_D[GP][2]) / k2;
         } else {
             w0 = T(WENO_D[GP][0]) * (T(1) + k0);
             w1 = T(WENO_D[GP][1]) * (T(1) + k1);
             w2 = T(WENO_D[GP][2]) * (T(1) + k2);
         }
         const T sum = w0 + w1 + w2;
-        if constexpr (sizeof(T) == 8) {
-            w0 = w0 / sum; w1 = w1 / sum; w2 = w2 / sum;
-            return w0 * h0 + w1 * h1 + w2 * h2;
+        w0 = w0 / sum; w1 = w1 / sum; w2 = w2 / sum;
+        return w0 * h0 + w1 * h1 + w2 * h2;
+    }
+};
+
+// Float policy.  Candidate k is the quadratic with the stencil's cell averages,
+//   p_k(x) = c + u_k x / 2 + t_k (x^2 - 1/12) / 2   (x in cell widths from the cell centre),
+// whose first- and second-difference forms u_k, t_k are exactly the linear forms inside the
+// smoothness indicator beta_k = u_k^2 / 4 + 13 t_k^2 / 12.  With the node-independent
+// factors q_k (alpha_k = d_k q_k) folded in once per stencil, a node costs
+//   c + (sum_k d_k (A u_k q_k + B t_k q_k)) / (sum_k d_k q_k),  A = x/2, B = (x^2 - 1/12)/2,
+// the same value as the tabulated candidates (WENO_H) up to rounding.
+template <int SCHEME>
+struct Weno5<float, SCHEME> {
+    float c, pu[3], pt[3], q[3];
+
+    WFV_DEV Weno5(float a, float b, float cc, float d, float e) : c(cc) {
+        const float u0 = a - 4.0f * b + 3.0f * cc, t0 = a - 2.0f * b + cc;
+        const float u1 = d - b, t1 = b - 2.0f * cc + d;
+        const float u2 = -3.0f * cc + 4.0f * d - e, t2 = cc - 2.0f * d + e;
+        const float b0 = 0.25f * (u0 * u0) + float(13.0 / 12.0) * (t0 * t0);
+        const float b1 = 0.25f * (u1 * u1) + float(13.0 / 12.0) * (t1 * t1);
+        const float b2 = 0.25f * (u2 * u2) + float(13.0 / 12.0) * (t2 * t2);
+        if constexpr (SCHEME == 0) {
+            // WENO5threconstruction (include/Utility.hpp:48-61): alpha_k = d_k/(eps+beta_k)^2
+            const float eps = 1.0e-6f;
+            q[0] = __builtin_amdgcn_rcpf((eps + b0) * (eps + b0));
+            q[1] = __builtin_amdgcn_rcpf((eps + b1) * (eps + b1));
+            q[2] = __builtin_amdgcn_rcpf((eps + b2) * (eps + b2));
         } else {
-            return (w0 * h0 + w1 * h1 + w2 * h2) * __builtin_amdgcn_rcpf(sum);
+            // WENO5Zthreconstruction (include/Utility.hpp:261-280) on the min-max normalised
+            // stencil: the indicators are shift-invariant quadratic forms, so tau/(beta_k + eps)
+            // there equals tau_raw/(beta_k,raw + eps den^2) here; the floor keeps a constant
+            // stencil at q = 1 where eps den^2 underflows.
+            const float mx = wMax(wMax(wMax(a, b), wMax(e, d)), cc);
+            const float mn = wMin(wMin(wMin(a, b), wMin(e, d)), cc);
+            const float den = mx - mn + 1e-20f;
+            const float e2 = wMax(1.0e-10f * den * den, 1.17549435e-38f);
+            const float tau = fabsf(b0 - b2);
+            q[0] = tau * __builtin_amdgcn_rcpf(b0 + e2) + 1.0f;
+            q[1] = tau * __builtin_amdgcn_rcpf(b1 + e2) + 1.0f;
+            q[2] = tau * __builtin_amdgcn_rcpf(b2 + e2) + 1.0f;
         }
+        pu[0] = q[0] * u0; pu[1] = q[1] * u1; pu[2] = q[2] * u2;
+        pt[0] = q[0] * t0; pt[1] = q[1] * t1; pt[2] = q[2] * t2;
+    }
+
+    template <int GP>
+    WFV_DEV float node() const {
+        // Gauss-Lobatto nodes x = -1/2, -sqrt(5)/10, sqrt(5)/10, 1/2
+        constexpr double A = GP == 0 ? -0.25 : GP == 1 ? -0.11180339887498948 : GP == 2 ? 0.11180339887498948 : 0.25;
+        constexpr double B = (GP == 0 || GP == 3) ? 1.0 / 12.0 : -1.0 / 60.0;
+        constexpr double d0 = WENO_D[GP][0], d1 = WENO_D[GP][1], d2 = WENO_D[GP][2];
+        const float num = float(d0 * A) * pu[0] + float(d0 * B) * pt[0] + float(d1 * A) * pu[1] +
+                          float(d1 * B) * pt[1] + float(d2 * A) * pu[2] + float(d2 * B) * pt[2];
+        const float sum = float(d0) * q[0] + float(d1) * q[1] + float(d2) * q[2];
+        return c + num * __builtin_amdgcn_rcpf(sum);
     }
 };
 
