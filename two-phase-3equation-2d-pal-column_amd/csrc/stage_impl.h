@@ -190,47 +190,65 @@ struct FaceSide<PrecMixed> {
 };
 
 // Riemann flux through one face, integrated over its 4 Gauss-Lobatto nodes
-// (src/WENOFV.cpp:1707-1746 fluxes, 1764-1799 quadrature).  The node loop is rolled: the
-// node values rotate through registers so that one copy of the solver is emitted.
+// (src/WENOFV.cpp:1707-1746 fluxes, 1764-1799 quadrature).
 template <class PR, int FLUX, int NX, int NY>
 WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceSide<PR> &b, const double bb[4],
                        double width, double G[4], unsigned &err, bool check) {
-    using F = typename PR::F;
 #pragma unroll
     for (int v = 0; v < 4; ++v) G[v] = 0.0;
-    // node in register slot `slot` with quadrature weight GLW[g] width / 2
-    auto node = [&](int slot, int g) {
-        double ul[4], ur[4];
-        F fl[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            ul[v] = a.get(v, slot, ba[v]);
-            ur[v] = b.get(v, slot, bb[v]);
-        }
-        double pref;
-        riemann<F, FLUX, NX, NY>(P, ul, ur, fl, pref);
-        const double qw = GLW[g] * width / 2.0;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (check && isnan(double(fl[v]))) err |= WFV_FLAG_NAN_FLUX;
-            if constexpr (PR::exact) {
-                G[v] += double(fl[v]) * qw;
-            } else {  // add back the reference pressure of the normal momentum flux
-                const double fv = (v == (NX ? 2 : 3)) ? double(fl[v]) + pref : double(fl[v]);
-                G[v] += fv * qw;
-            }
-        }
-    };
     if constexpr (PR::exact) {
+        // node in register slot 0 with quadrature weight GLW[g] width / 2; the node loop is
+        // rolled, the node values rotating through the registers
 #pragma unroll 1
         for (int g = 0; g < 4; ++g) {
-            node(0, g);
+            double ul[4], ur[4], fl[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                ul[v] = a.get(v, 0, ba[v]);
+                ur[v] = b.get(v, 0, bb[v]);
+            }
+            double pref;
+            riemann<double, FLUX, NX, NY>(P, ul, ur, fl, pref);
+            const double qw = GLW[g] * width / 2.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (check && isnan(fl[v])) err |= WFV_FLAG_NAN_FLUX;
+                G[v] += fl[v] * qw;
+            }
             a.rotate();  // node g+1 moves to slot 0
             b.rotate();
         }
-    } else {  // the mixed solver is small enough to unroll without register rotation
+    } else {
+        // Mixed: unrolled; momenta as float base + float deviation, the face frame's
+        // (normal, tangential) momenta mapped back to (x, y); a NaN in any node's flux makes
+        // its quadrature sum NaN, so the sums are checked once per face.
+        constexpr int N = NX ? 2 : 3, T = NX ? 3 : 2;
+        const float ban = float(ba[N]), bat = float(ba[T]), bbn = float(bb[N]), bbt = float(bb[T]);
+        const float djn = float(bb[N] - ba[N]), djt = float(bb[T] - ba[T]);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) node(g, g);
+        for (int g = 0; g < 4; ++g) {
+            const SideF L = {a.r[0][g], a.r[1][g], ban + a.m[N - 2][g], bat + a.m[T - 2][g]};
+            const SideF R = {b.r[0][g], b.r[1][g], bbn + b.m[N - 2][g], bbt + b.m[T - 2][g]};
+            float fl[4];
+            double pref;
+            if constexpr (FLUX == 1) {
+                hllc_f(P, L, R, fl, pref);
+            } else {
+                const float jump[4] = {float(R.a - L.a), float(R.b - L.b), djn + (b.m[N - 2][g] - a.m[N - 2][g]),
+                                       djt + (b.m[T - 2][g] - a.m[T - 2][g])};
+                llf_f(P, L, R, jump, fl, pref);
+            }
+            const double qw = GLW[g] * width / 2.0;
+            G[0] += double(fl[0]) * qw;
+            G[1] += double(fl[1]) * qw;
+            G[N] += (double(fl[2]) + pref) * qw;
+            G[T] += double(fl[3]) * qw;
+        }
+        if (check) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (isnan(G[v])) err |= WFV_FLAG_NAN_FLUX;
+        }
     }
 }
 

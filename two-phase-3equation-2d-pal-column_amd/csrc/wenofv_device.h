@@ -223,6 +223,71 @@ WFV_DEV void riemann_hllc(const Phys &P, const double dl[4], const double dr[4],
     for (int r = 0; r < 4; ++r) F[r] = outside ? F_K[r] : S_K * (k * Us[r] - U_K[r]) + F_K[r];
 }
 
+// Mixed-policy node solvers on a face whose normal is x (NX = 1) or y.  Each side arrives as
+// its partial densities in double (they alone enter the equation of state) and its normal
+// and tangential momenta in float.  The same algorithms as above, with the products by the
+// normal's 0/1 components dropped, the left pressure offset (p_L - pref = 0) known, and the
+// star-state momenta rho_K (u_K + (S* - vn_K)) = rho_K S* and rho_K v_K = m_tK taken in their
+// reduced forms; only rounding differs.
+struct SideF {
+    double a, b;  // z1 rho1, z2 rho2
+    float mn, mt; // normal and tangential momentum
+};
+
+WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, float F[4], double &pref) {
+    Eos el, er;
+    const double UL[4] = {L.a, L.b, 0.0, 0.0}, UR[4] = {R.a, R.b, 0.0, 0.0};
+    eos_fast(P, UL, el);
+    eos_fast(P, UR, er);
+    pref = el.p;
+    const float rho_L = float(el.rho), rho_R = float(er.rho);
+    const float vn_L = L.mn * __builtin_amdgcn_rcpf(rho_L), vn_R = R.mn * __builtin_amdgcn_rcpf(rho_R);
+    const float pre_R = float(er.p - el.p), c_L = float(el.c), c_R = float(er.c);
+    const float aL = float(L.a), bL = float(L.b), aR = float(R.a), bR = float(R.b);
+    // fluxes [z1rho1, z2rho2, normal momentum, tangential momentum]
+    const float FL[4] = {aL * vn_L, bL * vn_L, L.mn * vn_L, L.mt * vn_L};
+    const float FR[4] = {aR * vn_R, bR * vn_R, R.mn * vn_R + pre_R, R.mt * vn_R};
+    const float sL = __builtin_amdgcn_sqrtf(rho_L), sR = __builtin_amdgcn_sqrtf(rho_R);
+    const float is = __builtin_amdgcn_rcpf(sL + sR);
+    const float vn_roe = (vn_L * sL + vn_R * sR) * is;
+    const float dvn = vn_R - vn_L;
+    const float c2_roe = (c_L * c_L * sL + c_R * c_R * sR) * is + (0.5f * sL * sR * is * is) * (dvn * dvn);
+    const float c_roe = __builtin_amdgcn_sqrtf(c2_roe);
+    const float S_L = __builtin_fminf(vn_L - c_L, vn_roe - c_roe);
+    const float S_R = __builtin_fmaxf(vn_R + c_R, vn_roe + c_roe);
+    const float qL = (S_L - vn_L) * rho_L, qR = (S_R - vn_R) * rho_R;
+    const float S_star = (qL * vn_L - qR * vn_R + pre_R) * __builtin_amdgcn_rcpf(qL - qR);
+    const bool left = S_star >= 0.0f;
+    const bool outside = left ? (S_L >= 0.0f) : (S_R <= 0.0f);
+    const float S_K = left ? S_L : S_R, vn_K = left ? vn_L : vn_R, rho_K = left ? rho_L : rho_R;
+    const float U_K[4] = {left ? aL : aR, left ? bL : bR, left ? L.mn : R.mn, left ? L.mt : R.mt};
+    const float k = (S_K - vn_K) * __builtin_amdgcn_rcpf(S_K - S_star);
+    const float Us[4] = {U_K[0], U_K[1], rho_K * S_star, U_K[3]};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float FK = left ? FL[r] : FR[r];
+        F[r] = outside ? FK : S_K * (k * Us[r] - U_K[r]) + FK;
+    }
+}
+
+// jump = right minus left state, formed by the caller from exact double differences of the
+// bases plus the float deviations
+WFV_DEV void llf_f(const Phys &P, const SideF &L, const SideF &R, const float jump[4], float F[4], double &pref) {
+    Eos el, er;
+    const double UL[4] = {L.a, L.b, 0.0, 0.0}, UR[4] = {R.a, R.b, 0.0, 0.0};
+    eos_fast(P, UL, el);
+    eos_fast(P, UR, er);
+    pref = el.p;
+    const float rho_L = float(el.rho), rho_R = float(er.rho);
+    const float vn_L = L.mn * __builtin_amdgcn_rcpf(rho_L), vn_R = R.mn * __builtin_amdgcn_rcpf(rho_R);
+    const float pre_R = float(er.p - el.p);
+    const float ws = __builtin_fmaxf(fabsf(vn_L) + float(el.c), fabsf(vn_R) + float(er.c));
+    const float FL[4] = {float(L.a) * vn_L, float(L.b) * vn_L, L.mn * vn_L, L.mt * vn_L};
+    const float FR[4] = {float(R.a) * vn_R, float(R.b) * vn_R, R.mn * vn_R + pre_R, R.mt * vn_R};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) F[r] = 0.5f * (FL[r] + FR[r] - ws * jump[r]);
+}
+
 // getRiemannFlux dispatch (include/Equations.hpp:329-360), resolved at compile time
 template <class T, int FLUX, int NX, int NY>
 WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], T F[4], double &pref) {
