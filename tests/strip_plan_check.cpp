@@ -1,0 +1,78 @@
+// Host-side check of the stage kernels' work decomposition (strip_plan, segment_of in
+// wenofv_kernels.h, the functions the kernels call): over many grid shapes, rows per strip
+// and both stencil halos, every cell is owned by exactly one lane of one wavefront; each
+// segment lies inside its wavefront; and every lane an owned cell depends on (the x-stencil
+// lanes of the reconstructions on both sides of its faces, the lanes solving its east and
+// west faces) sits in the same segment, on the adjacent column, with the same rows.
+// Prints "ok <cases>" or the first failure and exits 1.
+#include <cstdio>
+#include <vector>
+
+#include "wenofv_kernels.h"
+
+using wfv::Segment;
+using wfv::StripPlan;
+
+static int fail(const char *what, int nx, int ny, int ty, int halo, int strip, int lane) {
+    std::printf("FAIL %s nx=%d ny=%d ty=%d halo=%d strip=%d lane=%d\n", what, nx, ny, ty, halo, strip, lane);
+    return 1;
+}
+
+static int check(int nx, int ny, int ty, int halo) {
+    const int L = wfv::WAVE_LANES, R = halo - 1, SW = L - 2 * halo;
+    const StripPlan sp = wfv::strip_plan(nx, ny, ty, halo);
+    std::vector<int> count((size_t)nx * ny, 0);
+    for (int strip = 0; strip < sp.total; ++strip) {
+        Segment seg[64];
+        for (int lane = 0; lane < L; ++lane) seg[lane] = wfv::segment_of(sp, strip, lane, ty, ny, halo);
+        for (int lane = 0; lane < L; ++lane) {
+            const Segment &g = seg[lane];
+            if (g.segw < 1 || g.segw > L || g.j < 0 || g.j >= g.segw || lane - g.j + g.segw > L)
+                return fail("segment outside the wavefront", nx, ny, ty, halo, strip, lane);
+            // a segment with rows is wide enough for its stencils; the kernel's stencil
+            // (recon) and face lanes of any segment read only lanes of that segment
+            if (g.y1 > g.y0 && g.segw < 2 * halo + 1) return fail("narrow segment with rows", nx, ny, ty, halo, strip, lane);
+            const bool recon = g.j >= R && g.j <= g.segw - 1 - R, xface = g.j >= R && g.j <= g.segw - 2 - R;
+            if ((recon && (g.j - R < 0 || g.j + R > g.segw - 1)) || (xface && g.j + 1 > g.segw - 1))
+                return fail("stencil lane reads outside its segment", nx, ny, ty, halo, strip, lane);
+            if (g.y0 < 0 || g.y0 > g.y1 || g.y1 > ny || g.y1 - g.y0 > g.nrow_w)
+                return fail("rows", nx, ny, ty, halo, strip, lane);
+            const int xc = g.x0 - halo + g.j;
+            const bool owned = g.j >= halo && g.j < halo + SW && xc < nx && g.y1 > g.y0;
+            if (!owned) continue;
+            // lanes j-1 and j solve the west and east faces (xface lanes); lanes j-1..j+1
+            // reconstruct (recon lanes) from x-stencils j'-R..j'+R
+            for (int d = -1; d <= 1; ++d) {
+                const int jj = g.j + d;
+                if (jj - R < 0 || jj + R > g.segw - 1) return fail("stencil leaves the segment", nx, ny, ty, halo, strip, lane);
+                if (d <= 0 && !(jj >= R && jj <= g.segw - 2 - R)) return fail("face lane", nx, ny, ty, halo, strip, lane);
+            }
+            for (int d = -R - 1; d <= R + 1; ++d) {
+                const Segment &n = seg[lane + d];
+                if (n.y0 != g.y0 || n.y1 != g.y1 || n.x0 - halo + n.j != xc + d)
+                    return fail("neighbour lane", nx, ny, ty, halo, strip, lane);
+            }
+            for (int y = g.y0; y < g.y1; ++y) ++count[(size_t)y * nx + xc];
+        }
+    }
+    for (size_t i = 0; i < count.size(); ++i)
+        if (count[i] != 1) return fail("cell not owned exactly once", nx, ny, ty, halo, (int)(i / nx), (int)(i % nx));
+    return 0;
+}
+
+int main() {
+    const int nxs[] = {1, 5, 6, 7, 16, 20, 25, 26, 27, 31, 32, 50, 56, 57, 58, 59, 63, 64, 70, 100,
+                       112, 115, 116, 117, 128, 170, 174, 200, 256, 300, 512, 1000, 1024};
+    const int nys[] = {1, 2, 3, 7, 16, 33, 64, 100, 128, 257};
+    const int tys[] = {1, 2, 3, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+    int cases = 0;
+    for (int halo : {3, 4})
+        for (int nx : nxs)
+            for (int ny : nys)
+                for (int ty : tys) {
+                    if (check(nx, ny, ty, halo)) return 1;
+                    ++cases;
+                }
+    std::printf("ok %d\n", cases);
+    return 0;
+}

@@ -314,21 +314,21 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr int VU = EX ? 1 : 4;
     const int lane = threadIdx.x & (LANES - 1);
 
-    if (strip >= A.strips_x * A.strips_y) return;
+    const StripPlan &sp = A.sp;
+    if (strip >= sp.total) return;
     if (A.active && *A.active == 0) return;  // device-side time loop already finished
-    const int sx = strip % A.strips_x, sy = strip / A.strips_x;
     const int nx = A.nx, ny = A.ny;
     const size_t nc = (size_t)nx * ny;
-    const int x0 = sx * SW;
-    const int y0 = sy * A.ty;
-    const int y1 = min(y0 + A.ty, ny);
+    // this lane's segment (segment_of); stencils and face exchanges stay inside it
+    const Segment sg = segment_of(sp, strip, lane, A.ty, ny, HALO);
+    const int x0 = sg.x0, y0 = sg.y0, y1 = sg.y1, j = sg.j, segw = sg.segw, nrow_w = sg.nrow_w;
     const int nrow = y1 - y0;
-    const int xc = x0 - HALO + lane;
+    const int xc = x0 - HALO + j;
     bool mu;
     const int cx = map_index(xc, nx, A.bc, mu);
-    const bool owned = lane >= HALO && lane < HALO + SW && xc < nx;
-    const bool recon_lane = lane >= R && lane <= LANES - 1 - R;
-    const bool xface_lane = lane >= R && lane <= LANES - 2 - R;
+    const bool owned = j >= HALO && j < HALO + SW && xc < nx && nrow > 0;
+    const bool recon_lane = j >= R && j <= segw - 1 - R;
+    const bool xface_lane = j >= R && j <= segw - 2 - R;
     const Phys P = A.P;
     const bool bp = A.bp != 0;
 
@@ -378,8 +378,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         top_base[v] = 0.0;
     }
 
-    for (int it = 0; it <= nrow + 1; ++it) {
+    for (int it = 0; it <= nrow_w + 1; ++it) {
         const int r = y0 - 1 + it;
+        const bool live = it <= nrow + 1;  // this lane's segment still has rows
         if (it > 0) {
 #pragma unroll
             for (int k = 0; k < NW - 1; ++k)
@@ -748,7 +749,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             const bool interior_row = it <= nrow;
             // ---- 3. face below row r: Riemann problem (top of r-1 | bottom of r)
             double G[4] = {0.0, 0.0, 0.0, 0.0};
-            if (owned) {
+            if (owned && live) {
                 FS top_prev;
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
@@ -801,7 +802,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             }
             // ---- 5. complete row r-1: -north face, source, RK combination (src/WENOFV.cpp:1816-1827,
             //      1831-1872, 466-471)
-            if (it >= 2 && owned) {
+            if (it >= 2 && owned && live) {
                 const int rr = r - 1;
                 const double dyp = A.dyC[rr];
                 const double idyp = EX ? 0.0 : 1.0 / dyp;
@@ -915,8 +916,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.flux = FLUX;
     A.bp = B.bp;
     A.ty = I.ty;
-    A.strips_x = I.strips_x;
-    A.strips_y = I.strips_y;
+    A.sp = I.sp;
     A.flags = &ctl->flags;
     A.dtmin = B.want_dtmin ? &ctl->dtmin_bits : nullptr;
     A.faces_out = nullptr;

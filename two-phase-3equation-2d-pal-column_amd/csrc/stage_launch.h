@@ -8,7 +8,7 @@ namespace wfv {
 // one wavefront per workgroup: waves spread evenly over the CUs
 template <class PR, int MINW, int RECON, int SCHEME, int FLUX>
 hipError_t launch_stage_t(const StageArgs &A, hipStream_t s) {
-    const int nstrip = A.strips_x * A.strips_y;
+    const int nstrip = A.sp.total;
     hipLaunchKernelGGL((stage_kernel<PR, RECON, SCHEME, FLUX, MINW>), dim3(nstrip), dim3(LANES), 0, s, A);
     return hipGetLastError();
 }

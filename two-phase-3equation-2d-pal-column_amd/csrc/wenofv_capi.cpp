@@ -55,10 +55,8 @@ StageArgs base_args(wfv_ctx *c) {
     A.flux = c->opt.riemann_flux;
     A.bp = c->opt.pp_limiter;
     A.ty = c->ty;
-    const int sw = strip_width(c->opt.scheme, c->opt.reconstruction);
-    A.strips_x = (c->nx + sw - 1) / sw;
+    A.sp = strip_plan(c->nx, c->ny, c->ty, strip_halo(c->opt.scheme, c->opt.reconstruction));
     A.dx4 = std::pow(c->dX, 4);  // WENO-EXP's threshold, as the reference's pow(m_deltaX, 4)
-    A.strips_y = (c->ny + c->ty - 1) / c->ty;
     A.flags = &c->d_ctl->flags;
     return A;
 }
@@ -479,11 +477,9 @@ int batch_layout(wfv_batch *b) {
     for (int i : order) {
         BatchInst &I = b->hinst[i];
         I.ty = b->ty[i];
-        const int sw = strip_width(b->opt.scheme, b->opt.reconstruction);
-        I.strips_x = (I.nx + sw - 1) / sw;
-        I.strips_y = (I.ny + I.ty - 1) / I.ty;
+        I.sp = strip_plan(I.nx, I.ny, I.ty, strip_halo(b->opt.scheme, b->opt.reconstruction));
         I.item_begin = (int)item_inst.size();
-        for (int k = 0; k < I.strips_x * I.strips_y; ++k) item_inst.push_back(i);
+        for (int k = 0; k < I.sp.total; ++k) item_inst.push_back(i);
     }
     b->total_items = (int)item_inst.size();
     (void)hipFree(b->d_item_inst);

@@ -186,5 +186,8 @@ hipError_t launch_step_control(StepCtl *S, hipStream_t s) {
 int strip_width(int scheme, int recon) {
     return (scheme == 2 || recon == 4) ? Stencil<2, 0>::SW : Stencil<0, 0>::SW;
 }
+int strip_halo(int scheme, int recon) {
+    return (scheme == 2 || recon == 4) ? Stencil<2, 0>::HALO : Stencil<0, 0>::HALO;
+}
 
 }  // namespace wfv

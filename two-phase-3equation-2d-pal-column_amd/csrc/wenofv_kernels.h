@@ -45,6 +45,82 @@ struct Stencil {
     static constexpr int SW = WAVE_LANES - 2 * HALO;  // owned columns per wavefront
 };
 
+// Wavefronts of one grid.  Full strips own SW = 64 - 2*halo columns each and march `ty` rows
+// (nfull x sy of them).  The remaining nx % SW columns form a narrow strip of seg = width +
+// 2*halo lanes; when K >= 2 such segments fit in a wavefront, that strip is cut into row
+// segments of tyr = ceil(ty / K) rows and K of them share a wavefront, so a grid a little
+// wider than a multiple of SW does not leave most lanes of its last strip idle.
+struct StripPlan {
+    int nfull, sy;  // full strips: nfull columns x sy rows of strips
+    int seg, K;     // lanes per remainder segment and segments per wavefront (K = 1, seg = 64: unpacked)
+    int tyr;        // rows per remainder segment
+    int total;      // wavefronts of the grid
+};
+__host__ __device__ inline StripPlan strip_plan(int nx, int ny, int ty, int halo) {
+    StripPlan p;
+    const int sw = WAVE_LANES - 2 * halo;
+    p.nfull = nx / sw;
+    p.sy = (ny + ty - 1) / ty;
+    const int wr = nx - p.nfull * sw;
+    p.seg = WAVE_LANES;
+    p.K = 1;
+    p.tyr = ty;
+    int nrem = 0;
+    if (wr > 0) {
+        const int k = WAVE_LANES / (wr + 2 * halo);
+        if (k >= 2) {
+            p.seg = wr + 2 * halo;
+            p.K = k;
+            p.tyr = (ty + k - 1) / k;
+            const int nseg = (ny + p.tyr - 1) / p.tyr;
+            nrem = (nseg + k - 1) / k;
+        } else {
+            nrem = p.sy;
+        }
+    }
+    p.total = p.nfull * p.sy + nrem;
+    return p;
+}
+
+// The segment of lane `lane` in wavefront `strip`: a full strip is one segment of 64 lanes;
+// a wavefront of the packed remainder strip holds K segments of sp.seg lanes, each with its
+// own rows (segments past the last row get y0 = y1), and the 64 - K*seg lanes after them form
+// a tail segment of that width with no rows, too narrow for any stencil.  Lane j of a
+// segment holds column x0 - halo + j.  nrow_w is the wavefront's row count (its longest
+// segment).
+struct Segment {
+    int x0, y0, y1, j, segw, nrow_w;
+};
+__host__ __device__ inline Segment segment_of(const StripPlan &sp, int strip, int lane, int ty, int ny, int halo) {
+    const int sw = WAVE_LANES - 2 * halo;
+    const int nfs = sp.nfull * sp.sy;
+    Segment g;
+    if (strip < nfs) {
+        g.x0 = (strip % sp.nfull) * sw;
+        g.y0 = (strip / sp.nfull) * ty;
+        g.y1 = g.y0 + ty < ny ? g.y0 + ty : ny;
+        g.j = lane;
+        g.segw = WAVE_LANES;
+        g.nrow_w = g.y1 - g.y0;
+    } else {
+        g.x0 = sp.nfull * sw;
+        const int k = lane / sp.seg;
+        if (k < sp.K) {
+            g.segw = sp.seg;
+            g.j = lane - k * sp.seg;
+            const int s0 = (strip - nfs) * sp.K + k;
+            g.y0 = s0 * sp.tyr < ny ? s0 * sp.tyr : ny;
+            g.y1 = g.y0 + sp.tyr < ny ? g.y0 + sp.tyr : ny;
+        } else {  // tail
+            g.segw = WAVE_LANES - sp.K * sp.seg;
+            g.j = lane - sp.K * sp.seg;
+            g.y0 = g.y1 = ny;
+        }
+        g.nrow_w = sp.tyr;
+    }
+    return g;
+}
+
 // One RK stage: Uo = (a*Un + b*U) + (c*dt)*rhs(U).  Un == nullptr stands for Un = U.
 struct StageArgs {
     const double *U;      // stage input, SoA [4][ny][nx]
@@ -63,7 +139,7 @@ struct StageArgs {
     const double *ghost;  // fixed ghost ring (boundary 3), SoA [4][ring_size]
     int nx, ny, bc, flux, bp;
     int ty;               // rows per strip
-    int strips_x, strips_y;
+    StripPlan sp;         // wavefronts of the grid (strip_plan)
     unsigned *flags;                 // error bits (WFV_FLAG_*)
     unsigned long long *dtmin;       // if set: CFL minimum of the output state (bit pattern)
     double *faces_out;               // parity hook: limited face values [cell][side][v][g]
@@ -105,7 +181,7 @@ struct BatchInst {
     double dx4;                  // pow(dX, 4) (WENO-EXP)
     const double *ghost;         // fixed ghost ring (boundary 3) or nullptr
     int nx, ny, bc, ty;
-    int strips_x, strips_y;
+    StripPlan sp;                // wavefronts of the grid (strip_plan)
     int item_begin;              // first item of this instance in the flattened list
     int pad;
 };
@@ -128,5 +204,6 @@ hipError_t launch_batch_dt(const BatchInst *inst, int n, int max_cells, hipStrea
 hipError_t launch_dt(const DtArgs &A, hipStream_t s);
 hipError_t launch_step_control(StepCtl *S, hipStream_t s);
 int strip_width(int scheme, int recon);  // owned columns per wavefront
+int strip_halo(int scheme, int recon);   // halo columns on each side of a strip
 
 }  // namespace wfv
