@@ -205,13 +205,16 @@ def run_batch(a, torch, wenofv, cases):
     }
     prof = f"batch_stage_{'fp64' if runs[0][1] == 0 else 'mixed'}_batch256.md"
     out["roofline"] = roofline(cells, results["mixed"]["ms"] / (3 * steps), profiled(prof, "traffic_bytes_per_launch"))
-    # the bound that applies: VALU issue (rocprofv3 counters of the same kernel and workload)
-    out["compute"] = {"bound": "valu", "issue_frac": profiled(prof, "valu_issue_frac"),
+    # the bound that applies: VALU throughput (rocprofv3 counters of the same kernel and workload)
+    out["compute"] = {"bound": "valu", "busy_frac": profiled(prof, "valu_busy_frac"),
+                      "issue_frac": profiled(prof, "valu_issue_frac"),
                       "lane_instructions_per_cell_stage": profiled(prof, "lane_instructions_per_cell_stage"),
                       "profile_stage_ms": (profiled(prof, "stage_kernel_avg_ns") or 0) / 1e6 or None,
                       "bench_stage_ms": results["mixed"]["ms"] / (3 * steps), "source": f"profiles/{prof}",
-                      "note": "issue_frac = SQ_INSTS_VALU per SIMD cycle / 0.25 (a wave64 VALU instruction "
-                              "occupies a SIMD for 4 cycles)"}
+                      "note": "busy_frac = SQ_ACTIVE_INST_VALU quad-cycles / (SIMDs x duration x clock / 4): the "
+                              "share of VALU time the kernel keeps busy, multi-cycle transcendental and FP64 "
+                              "instructions included; issue_frac = SQ_INSTS_VALU per SIMD cycle / 0.25 (a "
+                              "wave64 VALU instruction occupies a SIMD for 4 cycles)"}
     if not a.no_fp64:
         plain_rate, plain_ms = plain_fp64_per_instance(torch, wenofv, wls, opts, stream, a.plain_steps)
         out["fp64_plain_gpu"] = {

@@ -469,7 +469,8 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                 for (int t = 0; t < 5; ++t)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) sw[t][v] = sv(t, v);
+                    // DEV: the centre's deviation from itself is 0 (a literal the compiler folds)
+                    for (int v = 0; v < 4; ++v) sw[t][v] = (DEV && t == 2) ? W(0) : sv(t, v);
 #pragma unroll VU
                 for (int v = 0; v < 4; ++v) {  // variable v sits in slot 0
                     W s5[5] = {sw[0][0], sw[1][0], sw[2][0], sw[3][0], sw[4][0]};
