@@ -22,7 +22,10 @@
 //      neighbour lane's west face values exchanged through LDS,
 //   5. completes the RHS of the cell of row r-1, applies the RK combination and stores.
 // Lanes 3..60 (4..59) own output cells; lanes 2 and 61 (3 and 60) recompute the neighbouring
-// strips' edge cells, whose face values the strip's boundary fluxes need.
+// strips' edge cells, whose face values the strip's boundary fluxes need.  The columns left
+// over after the full strips (nx mod 58) form a narrower strip, which is cut into row
+// segments packed several per wavefront (strip_plan, segment_of in wenofv_kernels.h); every
+// lane then works relative to its segment.
 #pragma once
 #include <hip/hip_runtime.h>
 
