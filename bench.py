@@ -14,6 +14,7 @@ Workloads
                       and the reference state for the error check (relative L1 per field).
   settings_cfg        the reference's shipped config/settings.cfg (SMOOTH 100x100, WENO-Z,
                       characteristic, LLF, CFL 0.5) to the case's final time, --precision
+  smooth_1024         the same options and case at 1024x1024, 400 steps, --precision [--compare]
   tanh_interface      128x128 periodic smooth interface, WENO-JS + LLF, to t = 1, --precision
   liquid_column       1024x512 single case (walls, WENO-Z + HLLC + limiter), --precision
   perturbed_column    4096x2048 single case, --precision [--compare]
@@ -292,7 +293,17 @@ def run_single(a, torch, wenofv, cases):
         w.geometry_ranks = 4
         return w
 
+    def smooth_1024():
+        # the shipped config/settings.cfg's scheme options (WENO-Z, characteristic, LLF, no
+        # limiter, RK3, CFL 0.5) on its case (TESTCASE 0, SMOOTH) at 1024x1024, 400 steps
+        kw, _ = wenofv.reference_case(0)
+        U0 = wenofv.reference_initial_state(0, 1024, 1024, geometry_ranks=1)
+        prm = {k: kw[k] for k in ("rho10", "rho20", "sound1", "sound2", "preb")}
+        return cases.Workload("smooth_1024", 1024, 1024, kw["domain"], prm, U0, kw["bc"],
+                              dict(scheme=1, recon=1, flux=0, bp=0, rk=2, cfl=0.5, precision=a.precision), steps=400)
+
     makers = {"settings_cfg": settings_cfg,
+              "smooth_1024": smooth_1024,
               "tanh_interface": lambda: cases.tanh_interface(),
               "liquid_column": lambda: cases.liquid_column(precision=a.precision),
               "perturbed_column": lambda: cases.perturbed_column(precision=a.precision),

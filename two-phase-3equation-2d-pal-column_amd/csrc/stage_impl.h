@@ -111,9 +111,9 @@ WFV_DEV void from_prim(const Phys &P, const W w[4], F out[4]) {
     out[0] = F(z1rho1); out[1] = F(z2rho2); out[2] = F(rho * double(w[2])); out[3] = F(rho * double(w[3]));
 }
 
-template <class W, int SCHEME>
+template <bool EX, class W, int SCHEME>
 WFV_DEV void recon4(const W s[5], W out[4]) {
-    Weno5<W, SCHEME> st(s[0], s[1], s[2], s[3], s[4]);
+    WenoSel<EX, W, SCHEME> st(s[0], s[1], s[2], s[3], s[4]);
     out[0] = st.template node<0>();
     out[1] = st.template node<1>();
     out[2] = st.template node<2>();
@@ -261,10 +261,10 @@ using ReconT = typename std::conditional<RECON == 1 || Stencil<SCHEME, RECON>::R
 
 // the 4 Gauss-Lobatto node values of variable v from a 7-point stencil: WENO-EXP (SCHEME 2),
 // or for primitive3 THINC on z1 and WENO-Z on p, u, v (src/WENOFV.cpp:1287-1329)
-template <int SCHEME, int RECON>
+template <bool EX, int SCHEME, int RECON>
 WFV_DEV void recon7(const double s[7], int v, double dx4, double out[4]) {
     if (RECON == 4 && v != 0) {
-        Weno5<double, 1> st(s[1], s[2], s[3], s[4], s[5]);
+        WenoSel<EX, double, 1> st(s[1], s[2], s[3], s[4], s[5]);
         out[0] = st.template node<0>(); out[1] = st.template node<1>();
         out[2] = st.template node<2>(); out[3] = st.template node<3>();
     } else if (RECON == 4) {
@@ -278,10 +278,10 @@ WFV_DEV void recon7(const double s[7], int v, double dx4, double out[4]) {
     }
 }
 // the same at the 12 boundary nodes' x positions: nodes 0 and 3 always, 1 and 2 if `all`
-template <int SCHEME, int RECON>
+template <bool EX, int SCHEME, int RECON>
 WFV_DEV void recon7_x(const double s[7], int v, double dx4, bool all, double out[4]) {
     if (RECON == 4 && v != 0) {
-        Weno5<double, 1> st(s[1], s[2], s[3], s[4], s[5]);
+        WenoSel<EX, double, 1> st(s[1], s[2], s[3], s[4], s[5]);
         out[0] = st.template node<0>(); out[3] = st.template node<3>();
         if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
     } else if (RECON == 4) {
@@ -408,7 +408,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             };
             if constexpr (R == 3 && RECON == 1) {
                 double Lm[16], Rm[16], pc[7][4], yc[4][4], yv[4];
-                eigen_LR<double, false>(P, Uc, Lm, Rm);
+                eigen_LR<double, false, !EX>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int t = 0; t < 7; ++t) {
                     double u4[4] = {sv(t, 0), sv(t, 1), sv(t, 2), sv(t, 3)};
@@ -418,7 +418,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
                     const double s7[7] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0], pc[5][0], pc[6][0]};
                     double o[4];
-                    recon7<SCHEME, RECON>(s7, k, A.dx4, o);
+                    recon7<EX, SCHEME, RECON>(s7, k, A.dx4, o);
 #pragma unroll
                     for (int g = 0; g < 4; ++g) rotate_push(yc[g], o[g]);
 #pragma unroll
@@ -437,13 +437,13 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                     for (int t = 0; t < 7; ++t) s7[t] = sv(t, v);
                     double o[4];
-                    recon7<SCHEME, RECON>(s7, v, A.dx4, o);
+                    recon7<EX, SCHEME, RECON>(s7, v, A.dx4, o);
 #pragma unroll
                     for (int g = 0; g < 4; ++g) L.y[g * 4 + v][lane] = o[g];
                 }
             } else if constexpr (RECON == 1) {
                 W Lm[16], Rm[16], pc[5][4], yc[4][4], yv[4];
-                eigen_LR<W, false>(P, Uc, Lm, Rm);
+                eigen_LR<W, false, !EX>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int t = 0; t < 5; ++t) {
                     W u4[4] = {sv(t, 0), sv(t, 1), sv(t, 2), sv(t, 3)};
@@ -453,7 +453,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
                     W s5[5] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]};
                     W o[4];
-                    recon4<W, SCHEME>(s5, o);
+                    recon4<EX, W, SCHEME>(s5, o);
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         rotate_push(yc[g], o[g]);
@@ -478,7 +478,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int v = 0; v < 4; ++v) {  // variable v sits in slot 0
                     W s5[5] = {sw[0][0], sw[1][0], sw[2][0], sw[3][0], sw[4][0]};
                     W o[4];
-                    recon4<W, SCHEME>(s5, o);
+                    recon4<EX, W, SCHEME>(s5, o);
 #pragma unroll
                     for (int g = 0; g < 4; ++g) L.y[g * 4 + v][lane] = o[g];
 #pragma unroll
@@ -515,7 +515,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             W ug[4][4][4];  // Ugp[gx][gy][v] at the 12 boundary nodes
             if constexpr (R == 3 && RECON == 1) {
                 double Lm[16], Rm[16];
-                eigen_LR<double, true>(P, Uc, Lm, Rm);
+                eigen_LR<double, true, !EX>(P, Uc, Lm, Rm);
 #pragma unroll 1
                 for (int gy = 0; gy < 4; ++gy) {  // node row gy sits in slot 0 of ug[.][0]
                     double pc[7][4];
@@ -530,7 +530,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     for (int k = 0; k < 4; ++k) {
                         const double s7[7] = {pc[0][k], pc[1][k], pc[2][k], pc[3][k], pc[4][k], pc[5][k], pc[6][k]};
                         double o[4] = {0.0, 0.0, 0.0, 0.0};
-                        recon7_x<SCHEME, RECON>(s7, k, A.dx4, all, o);
+                        recon7_x<EX, SCHEME, RECON>(s7, k, A.dx4, all, o);
 #pragma unroll
                         for (int gx = 0; gx < 4; ++gx) uc[gx][k] = o[gx];
                     }
@@ -562,14 +562,14 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                         for (int t = 0; t < 7; ++t) s7[t] = xv(gy, v, t);
                         double o[4] = {0.0, 0.0, 0.0, 0.0};
-                        recon7_x<SCHEME, RECON>(s7, v, A.dx4, gy == 0 || gy == 3, o);
+                        recon7_x<EX, SCHEME, RECON>(s7, v, A.dx4, gy == 0 || gy == 3, o);
 #pragma unroll
                         for (int gx = 0; gx < 4; ++gx) rotate_push(ug[gx][gy], o[gx]);
                     }
                 }
             } else if constexpr (RECON == 1) {
                 W Lm[16], Rm[16];
-                eigen_LR<W, true>(P, Uc, Lm, Rm);
+                eigen_LR<W, true, !EX>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int gy = 0; gy < 4; ++gy) {
                     W pc[5][4];
@@ -581,7 +581,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     W uc[4][4];  // [gx][k]
 #pragma unroll 1
                     for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
-                        Weno5<W, SCHEME> st(pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]);
+                        WenoSel<EX, W, SCHEME> st(pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]);
                         rotate_push(uc[0], st.template node<0>());
                         rotate_push(uc[3], st.template node<3>());
                         if (gy == 0 || gy == 3) {
@@ -604,7 +604,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int gy = 0; gy < 4; ++gy)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        Weno5<W, SCHEME> st(xv(gy, v, 0), xv(gy, v, 1), xv(gy, v, 2), xv(gy, v, 3), xv(gy, v, 4));
+                        WenoSel<EX, W, SCHEME> st(xv(gy, v, 0), xv(gy, v, 1), xv(gy, v, 2), xv(gy, v, 3), xv(gy, v, 4));
                         ug[0][gy][v] = st.template node<0>();
                         ug[3][gy][v] = st.template node<3>();
                         if (gy == 0 || gy == 3) {
@@ -624,7 +624,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                             const int l = lane - 2 + t;
                             st5[t] = W(L.base[v][l] - bc[0]) + L.y[gy * 4 + v][l];
                         }
-                        Weno5<W, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
+                        WenoSel<EX, W, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
                         rotate_push(ug[0][gy], st.template node<0>());
                         rotate_push(ug[3][gy], st.template node<3>());
                         if (gy == 0 || gy == 3) {

@@ -11,6 +11,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "weno_constants.h"
 
 #define WFV_DEV __device__ __forceinline__
@@ -313,8 +315,63 @@ WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], T F[
 // ------------------------------------------------------------------ eigenvectors
 // getLEigenMatrix / getREigenMatrix (include/Equations.hpp:189-327), row-major 4x4,
 // built in double (they involve the equation of state) and used in W.
-template <class T, bool XDIR>
+template <class T, bool XDIR, bool FAST = false>
 WFV_DEV void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
+    if constexpr (FAST) {
+        // Mixed policy: the same matrices with the volume fraction from the division-free
+        // equation of state (z1 = N/(D+N), z2 = D/(D+N)) and every quotient as a product
+        // with a Newton-refined reciprocal; sc_k^2 uses 1 - zrho1/(zrho1+zrho2) =
+        // zrho2/(zrho1+zrho2), which does not cancel.
+        const double a = U[0], b = U[1];
+        const double t1 = a * P.c1_2, t2 = b * P.c2_2;
+        const double dq = P.q - (t2 - t1);
+        const double x = dq * dq + 4.0 * t1 * t2;
+        const double S = x * rsq_nr(fmax(x, 1.0e-300));
+        const double N = dq + S, D = 2.0 * t2, DN = D + N;
+        const double iDN = rcp_nr(DN);
+        const double z1 = N * iDN, z2 = D * iDN;
+        const double rho = a + b, irho = rcp_nr(rho);
+        const double c = DN * rsq_nr(rho * (N * N * rcp_nr(t1) + 2.0 * D));
+        const double u = U[2] * irho, v = U[3] * irho;
+        const double iz1 = rcp_nr(z1), iz2 = rcp_nr(z2);
+        const double zrho1 = t1 * iz1 * iz1, zrho2 = t2 * iz2 * iz2;
+        const double izz = rcp_nr(zrho1 + zrho2);
+        const double sc1_2 = P.c1_2 * iz1 * (zrho2 * izz);
+        const double sc2_2 = P.c2_2 * iz2 * (zrho1 * izz);
+        const double ic = rcp_nr(c), i2c2 = 0.5 * ic * ic, irc2 = irho * ic * ic;
+        const double d12 = sc1_2 - sc2_2, id12 = rcp_nr(d12);
+        const double un = XDIR ? u : v;
+        L[0] = T((sc1_2 + c * un) * i2c2);
+        L[1] = T((sc2_2 + c * un) * i2c2);
+        L[4] = T(-b * d12 * irc2);
+        L[5] = T(a * d12 * irc2);
+        L[6] = T(0); L[7] = T(0);
+        L[12] = T((sc1_2 - c * un) * i2c2);
+        L[13] = T((sc2_2 - c * un) * i2c2);
+        if (XDIR) {
+            L[2] = T(-0.5 * ic); L[3] = T(0);
+            L[8] = T(-v); L[9] = T(-v); L[10] = T(0); L[11] = T(1);
+            L[14] = T(0.5 * ic); L[15] = T(0);
+        } else {
+            L[2] = T(0); L[3] = T(-0.5 * ic);
+            L[8] = T(-u); L[9] = T(-u); L[10] = T(1); L[11] = T(0);
+            L[14] = T(0); L[15] = T(0.5 * ic);
+        }
+        R[0] = T(a * irho); R[4] = T(b * irho);
+        R[1] = T(-sc2_2 * id12); R[5] = T(sc1_2 * id12); R[9] = T(u); R[13] = T(v);
+        R[2] = T(0); R[6] = T(0);
+        R[3] = T(a * irho); R[7] = T(b * irho);
+        if (XDIR) {
+            R[8] = T(u - c); R[12] = T(v);
+            R[10] = T(0); R[14] = T(1);
+            R[11] = T(u + c); R[15] = T(v);
+        } else {
+            R[8] = T(u); R[12] = T(v - c);
+            R[10] = T(1); R[14] = T(0);
+            R[11] = T(u); R[15] = T(v + c);
+        }
+        return;
+    }
     Eos e;
     eos(P, U, e);
     const double rho = e.rho, u = U[2] / rho, v = U[3] / rho, c = e.c;
@@ -389,7 +446,7 @@ WFV_DEV void beta3(T a, T b, T c, T d, T e, T &b0, T &b1, T &b2) {
 // operation (include/Utility.hpp:5-95, 193-290).
 template <class T, int SCHEME>
 struct Weno5 {
-    static_assert(sizeof(T) == 8, "the float policy has its own specialisation below");
+    static_assert(sizeof(T) == 8, "the mixed policy uses WenoLin");
     T s0, s1, s2, s3, s4;
     T k0, k1, k2;  // JS: (eps+beta_k)^2; Z: tau/(beta_k+eps)
 
@@ -440,62 +497,74 @@ struct Weno5 {
     }
 };
 
-// Float policy.  Candidate k is the quadratic with the stencil's cell averages,
+// Mixed policy (float, and double for the forms it reconstructs in double).  Candidate k is
+// the quadratic with the stencil's cell averages,
 //   p_k(x) = c + u_k x / 2 + t_k (x^2 - 1/12) / 2   (x in cell widths from the cell centre),
 // whose first- and second-difference forms u_k, t_k are exactly the linear forms inside the
 // smoothness indicator beta_k = u_k^2 / 4 + 13 t_k^2 / 12.  With the node-independent
 // factors q_k (alpha_k = d_k q_k) folded in once per stencil, a node costs
 //   c + (sum_k d_k (A u_k q_k + B t_k q_k)) / (sum_k d_k q_k),  A = x/2, B = (x^2 - 1/12)/2,
-// the same value as the tabulated candidates (WENO_H) up to rounding.
-template <int SCHEME>
-struct Weno5<float, SCHEME> {
-    float c, pu[3], pt[3], q[3];
+// the same value as the tabulated candidates (WENO_H) up to rounding.  Reciprocals: the
+// hardware estimate in float, one Newton step on it in double.
+template <class T>
+WFV_DEV T rcp_mixed(T x) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+    else return rcp_nr(x);
+}
+template <class T, int SCHEME>
+struct WenoLin {
+    T c, pu[3], pt[3], q[3];
 
-    WFV_DEV Weno5(float a, float b, float cc, float d, float e) : c(cc) {
-        const float u0 = a - 4.0f * b + 3.0f * cc, t0 = a - 2.0f * b + cc;
-        const float u1 = d - b, t1 = b - 2.0f * cc + d;
-        const float u2 = -3.0f * cc + 4.0f * d - e, t2 = cc - 2.0f * d + e;
-        const float b0 = 0.25f * (u0 * u0) + float(13.0 / 12.0) * (t0 * t0);
-        const float b1 = 0.25f * (u1 * u1) + float(13.0 / 12.0) * (t1 * t1);
-        const float b2 = 0.25f * (u2 * u2) + float(13.0 / 12.0) * (t2 * t2);
+    WFV_DEV WenoLin(T a, T b, T cc, T d, T e) : c(cc) {
+        const T u0 = a - T(4) * b + T(3) * cc, t0 = a - T(2) * b + cc;
+        const T u1 = d - b, t1 = b - T(2) * cc + d;
+        const T u2 = T(-3) * cc + T(4) * d - e, t2 = cc - T(2) * d + e;
+        const T b0 = T(0.25) * (u0 * u0) + T(13.0 / 12.0) * (t0 * t0);
+        const T b1 = T(0.25) * (u1 * u1) + T(13.0 / 12.0) * (t1 * t1);
+        const T b2 = T(0.25) * (u2 * u2) + T(13.0 / 12.0) * (t2 * t2);
         if constexpr (SCHEME == 0) {
             // WENO5threconstruction (include/Utility.hpp:48-61): alpha_k = d_k/(eps+beta_k)^2
-            const float eps = 1.0e-6f;
-            q[0] = __builtin_amdgcn_rcpf((eps + b0) * (eps + b0));
-            q[1] = __builtin_amdgcn_rcpf((eps + b1) * (eps + b1));
-            q[2] = __builtin_amdgcn_rcpf((eps + b2) * (eps + b2));
+            const T eps = T(1.0e-6);
+            q[0] = rcp_mixed((eps + b0) * (eps + b0));
+            q[1] = rcp_mixed((eps + b1) * (eps + b1));
+            q[2] = rcp_mixed((eps + b2) * (eps + b2));
         } else {
             // WENO5Zthreconstruction (include/Utility.hpp:261-280) on the min-max normalised
             // stencil: the indicators are shift-invariant quadratic forms, so tau/(beta_k + eps)
-            // there equals tau_raw/(beta_k,raw + eps den^2) here; the floor keeps a constant
-            // stencil at q = 1 where eps den^2 underflows.
-            // (the normalisation's + 1e-20 offset matters only where eps den^2 is below the
-            // floor, so it is left out; the max/min chains become 3-operand instructions)
-            const float mx = wMax(wMax(wMax(wMax(a, b), cc), d), e);
-            const float mn = wMin(wMin(wMin(wMin(a, b), cc), d), e);
-            const float den = mx - mn;
-            const float e2 = wMax(1.0e-10f * den * den, 1.17549435e-38f);
-            const float tau = fabsf(b0 - b2);
-            q[0] = tau * __builtin_amdgcn_rcpf(b0 + e2) + 1.0f;
-            q[1] = tau * __builtin_amdgcn_rcpf(b1 + e2) + 1.0f;
-            q[2] = tau * __builtin_amdgcn_rcpf(b2 + e2) + 1.0f;
+            // there equals tau_raw/(beta_k,raw + eps den^2) here; the floor (the smallest
+            // normal number) keeps a constant stencil at q = 1 where eps den^2 underflows.
+            // The normalisation's + 1e-20 offset matters only below that floor and is left
+            // out; the max/min chains become 3-operand instructions.
+            const T mx = fmax(fmax(fmax(fmax(a, b), cc), d), e);
+            const T mn = fmin(fmin(fmin(fmin(a, b), cc), d), e);
+            const T den = mx - mn;
+            const T e2 = fmax(T(1.0e-10) * den * den, sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308));
+            const T tau = fabs(b0 - b2);
+            q[0] = tau * rcp_mixed(b0 + e2) + T(1);
+            q[1] = tau * rcp_mixed(b1 + e2) + T(1);
+            q[2] = tau * rcp_mixed(b2 + e2) + T(1);
         }
         pu[0] = q[0] * u0; pu[1] = q[1] * u1; pu[2] = q[2] * u2;
         pt[0] = q[0] * t0; pt[1] = q[1] * t1; pt[2] = q[2] * t2;
     }
 
     template <int GP>
-    WFV_DEV float node() const {
+    WFV_DEV T node() const {
         // Gauss-Lobatto nodes x = -1/2, -sqrt(5)/10, sqrt(5)/10, 1/2
         constexpr double A = GP == 0 ? -0.25 : GP == 1 ? -0.11180339887498948 : GP == 2 ? 0.11180339887498948 : 0.25;
         constexpr double B = (GP == 0 || GP == 3) ? 1.0 / 12.0 : -1.0 / 60.0;
         constexpr double d0 = WENO_D[GP][0], d1 = WENO_D[GP][1], d2 = WENO_D[GP][2];
-        const float num = float(d0 * A) * pu[0] + float(d0 * B) * pt[0] + float(d1 * A) * pu[1] +
-                          float(d1 * B) * pt[1] + float(d2 * A) * pu[2] + float(d2 * B) * pt[2];
-        const float sum = float(d0) * q[0] + float(d1) * q[1] + float(d2) * q[2];
-        return c + num * __builtin_amdgcn_rcpf(sum);
+        const T num = T(d0 * A) * pu[0] + T(d0 * B) * pt[0] + T(d1 * A) * pu[1] + T(d1 * B) * pt[1] +
+                      T(d2 * A) * pu[2] + T(d2 * B) * pt[2];
+        const T sum = T(d0) * q[0] + T(d1) * q[1] + T(d2) * q[2];
+        return c + num * rcp_mixed(sum);
     }
 };
+
+// the 5-point reconstruction of a policy: the reference's arithmetic in the all-FP64 policy,
+// the linear form in the mixed one
+template <bool EXACT, class T, int SCHEME>
+using WenoSel = typename std::conditional<EXACT, Weno5<T, SCHEME>, WenoLin<T, SCHEME>>::type;
 
 // ------------------------------------------------------------------ 7-point kernels
 // WENO-EXP (include/Utility.hpp:291-472) and the THINC kernel of the primitive3 form
