@@ -73,9 +73,26 @@ WFV_DEV void load_cell(const double *__restrict__ U, size_t nc, size_t off, bool
 
 
 
-// primitive variables (src/WENOFV.cpp:937-956 PRIMITIVE, 1097-1117 PRIMITIVE2)
-template <class W, int RECON>
+// primitive variables (src/WENOFV.cpp:937-956 PRIMITIVE, 1097-1117 PRIMITIVE2).  FAST (mixed
+// policy): the volume fraction and pressure of the division-free equation of state (with
+// N, D as in eos_fast: z1 = N/(D+N), rho1 = z1rho1 (D+N)/N, and p the phases' common
+// pressure), the velocities by one reciprocal of rho.
+template <class W, int RECON, bool FAST = false>
 WFV_DEV void to_prim(const Phys &P, const double U[4], W out[4]) {
+    if constexpr (FAST) {
+        const double a = U[0], b = U[1];
+        const double t1 = a * P.c1_2, t2 = b * P.c2_2;
+        const double dq = P.q - (t2 - t1);
+        const double x = dq * dq + 4.0 * t1 * t2;
+        const double N = dq + x * rsq_nr(fmax(x, 1.0e-300)), DN = 2.0 * t2 + N;
+        out[0] = W(N * rcp_nr(DN));
+        if constexpr (RECON == 2) out[1] = W(a * DN * rcp_nr(N));
+        else out[1] = W(P.preb + (0.5 * DN - P.c2_2 * P.rho20));
+        const double irho = rcp_nr(a + b);
+        out[2] = W(U[2] * irho);
+        out[3] = W(U[3] * irho);
+        return;
+    }
     Eos e;
     eos(P, U, e);
     const double z1 = e.z1;
@@ -92,20 +109,26 @@ WFV_DEV void to_prim(const Phys &P, const double U[4], W out[4]) {
     out[3] = W(U[3] / (U[0] + U[1]));
 }
 
-// conservative variables at a node (src/WENOFV.cpp:1007-1032, 1171-1189)
-template <class W, class F, int RECON>
+// conservative variables at a node (src/WENOFV.cpp:1007-1032, 1171-1189); FAST divides by
+// the sound speeds' squares through their reciprocals
+template <class W, class F, int RECON, bool FAST = false>
 WFV_DEV void from_prim(const Phys &P, const W w[4], F out[4]) {
     const double z1 = double(w[0]);
     double z1rho1, z2rho2;
     if constexpr (RECON == 2) {
         const double rho1 = double(w[1]);
-        z2rho2 = P.c1_2 / P.c2_2 * (rho1 - P.rho10) + P.rho20;
+        z2rho2 = (FAST ? P.c1_2 * P.ic2_2 : P.c1_2 / P.c2_2) * (rho1 - P.rho10) + P.rho20;
         z2rho2 = z2rho2 * (1.0 - z1);
         z1rho1 = z1 * rho1;
     } else {
         const double pr = double(w[1]);
-        z1rho1 = ((pr - P.preb) / P.c1_2 + P.rho10) * z1;
-        z2rho2 = ((pr - P.preb) / P.c2_2 + P.rho20) * (1.0 - z1);
+        if constexpr (FAST) {
+            z1rho1 = ((pr - P.preb) * P.ic1_2 + P.rho10) * z1;
+            z2rho2 = ((pr - P.preb) * P.ic2_2 + P.rho20) * (1.0 - z1);
+        } else {
+            z1rho1 = ((pr - P.preb) / P.c1_2 + P.rho10) * z1;
+            z2rho2 = ((pr - P.preb) / P.c2_2 + P.rho20) * (1.0 - z1);
+        }
     }
     const double rho = z1rho1 + z2rho2;
     out[0] = F(z1rho1); out[1] = F(z2rho2); out[2] = F(rho * double(w[2])); out[3] = F(rho * double(w[3]));
@@ -268,11 +291,11 @@ WFV_DEV void recon7(const double s[7], int v, double dx4, double out[4]) {
         out[0] = st.template node<0>(); out[1] = st.template node<1>();
         out[2] = st.template node<2>(); out[3] = st.template node<3>();
     } else if (RECON == 4) {
-        ExpThinc7 st(s, dx4);
+        ExpThinc7<!EX> st(s, dx4);
         out[0] = st.template node<0>(); out[1] = st.template node<1>();
         out[2] = st.template node<2>(); out[3] = st.template node<3>();
     } else {
-        WenoExp7 st(s, dx4);
+        WenoExp7<!EX> st(s, dx4);
         out[0] = st.template node<0>(); out[1] = st.template node<1>();
         out[2] = st.template node<2>(); out[3] = st.template node<3>();
     }
@@ -285,11 +308,11 @@ WFV_DEV void recon7_x(const double s[7], int v, double dx4, bool all, double out
         out[0] = st.template node<0>(); out[3] = st.template node<3>();
         if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
     } else if (RECON == 4) {
-        ExpThinc7 st(s, dx4);
+        ExpThinc7<!EX> st(s, dx4);
         out[0] = st.template node<0>(); out[3] = st.template node<3>();
         if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
     } else {
-        WenoExp7 st(s, dx4);
+        WenoExp7<!EX> st(s, dx4);
         out[0] = st.template node<0>(); out[3] = st.template node<3>();
         if (all) { out[1] = st.template node<1>(); out[2] = st.template node<2>(); }
     }
@@ -366,7 +389,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
         load_row(y0 - 1 - R + k, win[k]);
-        if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[k], pwin[k]);
+        if constexpr (RECON >= 2) to_prim<double, RECON, !EX>(P, win[k], pwin[k]);
     }
 
     // The north face values of row r-1 wait in LDS (L.top) for the face below row r; their
@@ -393,7 +416,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     if constexpr (RECON >= 2) pwin[k][v] = pwin[k + 1][v];
                 }
             load_row(r + R, win[NW - 1]);
-            if constexpr (RECON >= 2) to_prim<double, RECON>(P, win[NW - 1], pwin[NW - 1]);
+            if constexpr (RECON >= 2) to_prim<double, RECON, !EX>(P, win[NW - 1], pwin[NW - 1]);
         }
         const double *Uc = win[R];                          // cell (xc, r)
         const double *Bc = (RECON >= 2) ? pwin[R] : win[R];  // its reconstructed variables
@@ -643,7 +666,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                     for (int v = 0; v < 4; ++v) pv[v] = DEV ? Bc[v] + double(w[v]) : double(w[v]);
                     double cv[4];
-                    from_prim<double, double, RECON>(P, pv, cv);
+                    from_prim<double, double, RECON, !EX>(P, pv, cv);
 #pragma unroll
                     for (int v = 0; v < 4; ++v) f.set(v, g, cv[v], Uc[v]);
                 } else if constexpr (DEV || EX) {

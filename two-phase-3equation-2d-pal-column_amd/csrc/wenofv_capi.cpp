@@ -155,6 +155,8 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     c->P.c2_2 = cs->sound2 * cs->sound2;
     c->P.preb = cs->preb;
     c->P.q = cs->rho20 * c->P.c2_2 - cs->rho10 * c->P.c1_2;
+    c->P.ic1_2 = 1.0 / c->P.c1_2;
+    c->P.ic2_2 = 1.0 / c->P.c2_2;
     c->dX = (cs->xR - cs->xL) / c->nx;
     c->dY = (cs->yR - cs->yL) / c->ny;
     c->dxC.resize(c->nx);
@@ -625,6 +627,8 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         I.P.c2_2 = in.cs.sound2 * in.cs.sound2;
         I.P.preb = in.cs.preb;
         I.P.q = in.cs.rho20 * I.P.c2_2 - in.cs.rho10 * I.P.c1_2;
+        I.P.ic1_2 = 1.0 / I.P.c1_2;
+        I.P.ic2_2 = 1.0 / I.P.c2_2;
         I.dX = (in.cs.xR - in.cs.xL) / in.nx;
         I.dY = (in.cs.yR - in.cs.yL) / in.ny;
         I.dx4 = std::pow(I.dX, 4);

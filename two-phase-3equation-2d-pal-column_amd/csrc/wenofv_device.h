@@ -34,6 +34,7 @@ struct PrecMixed {
 // (c_k^2 = g_sound_k * g_sound_k, q = rho20*c2^2 - rho10*c1^2, include/Equations.hpp:86-92)
 struct Phys {
     double rho10, rho20, c1_2, c2_2, preb, q;
+    double ic1_2, ic2_2;  // 1/c1^2, 1/c2^2 (mixed policy)
 };
 
 // Gauss-Lobatto-4 weights (third_party/scTools/GaussQuadratureInRef.hpp:252-262)
@@ -579,10 +580,14 @@ WFV_DEV double cube_rn(double x) {
     return hi + (lo + e * x);
 }
 
+// FAST (mixed policy): the same formulas with each quotient a product by a Newton-refined
+// reciprocal, hardware min/max and a plain cube; the all-FP64 policy keeps the reference's
+// IEEE operations.
+
 // fitted linear weights, cubic in l2h2 (include/Utility.hpp:347-380); pow(l, 2) is l*l
-template <int GP>
+template <int GP, bool FAST = false>
 WFV_DEV void wexp_d(double l, double &d0, double &d1, double &d2) {
-    const double l2 = l * l, l3 = cube_rn(l);
+    const double l2 = l * l, l3 = FAST ? l2 * l : cube_rn(l);
     if constexpr (GP == 0) {
         d0 = 0.3 - 0.057142858 * l + 0.0071428571 * l2 - 0.00077200577 * l3;
         d1 = 0.6 + 0.078571429 * l - 0.0101190476 * l2 + 0.00112012987 * l3;
@@ -604,63 +609,83 @@ WFV_DEV void wexp_d(double l, double &d0, double &d1, double &d2) {
 
 // l2h2 = dif5/dif3 of the 6 cells biased toward the node (gp < 2: u_mmm..u_pp; else
 // u_mm..u_ppp), 0 where |dif3| < dx^4 (include/Utility.hpp:297-321)
+template <bool FAST = false>
 WFV_DEV double wexp_l2h2(const double *t, double dx4, bool clamp) {
     const double dif5 = (-t[0] + 5.0 * t[1] - 10.0 * t[2] + 10.0 * t[3] - 5.0 * t[4] + t[5]);
-    const double dif3 = (t[0] - 11.0 * t[1] + 28.0 * t[2] - 28.0 * t[3] + 11.0 * t[4] - t[5]) / 6.0;
+    const double d6 = t[0] - 11.0 * t[1] + 28.0 * t[2] - 28.0 * t[3] + 11.0 * t[4] - t[5];
+    const double dif3 = FAST ? d6 * (1.0 / 6.0) : d6 / 6.0;
     if (fabs(dif3) < dx4) return 0.0;
-    double l = dif5 / dif3;
+    double l = FAST ? dif5 * rcp_nr(dif3) : dif5 / dif3;
     if (clamp && fabs(l) > 1) l = 1;
     return l;
 }
 
 // the inner five cells scaled to [0, 1] by their min and max (include/Utility.hpp:393-401)
+template <bool FAST = false>
 WFV_DEV void wexp_norm(const double s[7], double n[5]) {
-    const double mx = stdMax(stdMax(stdMax(s[1], s[2]), stdMax(s[3], s[4])), s[5]);
-    const double mn = stdMin(stdMin(stdMin(s[1], s[2]), stdMin(s[3], s[4])), s[5]);
-    for (int k = 0; k < 5; ++k) n[k] = (s[1 + k] - mn) / (mx - mn + 1e-20);
+    if constexpr (FAST) {
+        const double mx = fmax(fmax(fmax(fmax(s[1], s[2]), s[3]), s[4]), s[5]);
+        const double mn = fmin(fmin(fmin(fmin(s[1], s[2]), s[3]), s[4]), s[5]);
+        const double r = rcp_nr(mx - mn + 1e-20);
+        for (int k = 0; k < 5; ++k) n[k] = (s[1 + k] - mn) * r;
+    } else {
+        const double mx = stdMax(stdMax(stdMax(s[1], s[2]), stdMax(s[3], s[4])), s[5]);
+        const double mn = stdMin(stdMin(stdMin(s[1], s[2]), stdMin(s[3], s[4])), s[5]);
+        for (int k = 0; k < 5; ++k) n[k] = (s[1 + k] - mn) / (mx - mn + 1e-20);
+    }
 }
 
 // weights -> value: w_k = d_k (1 + a_k), normalised, sum w_k h_k (candidates as WENO5)
-template <int GP>
+template <int GP, bool FAST = false>
 WFV_DEV double wexp_combine(const double s[7], double d0, double d1, double d2, double a0, double a1, double a2) {
     const double h0 = WENO_H[GP][0][0] * s[1] + WENO_H[GP][0][1] * s[2] + WENO_H[GP][0][2] * s[3];
     const double h1 = WENO_H[GP][1][0] * s[2] + WENO_H[GP][1][1] * s[3] + WENO_H[GP][1][2] * s[4];
     const double h2 = WENO_H[GP][2][0] * s[3] + WENO_H[GP][2][1] * s[4] + WENO_H[GP][2][2] * s[5];
     double w0 = d0 * a0, w1 = d1 * a1, w2 = d2 * a2;
     const double S = w0 + w1 + w2;
+    if constexpr (FAST) return (w0 * h0 + w1 * h1 + w2 * h2) * rcp_nr(S);
     w0 = w0 / S; w1 = w1 / S; w2 = w2 / S;
     return w0 * h0 + w1 * h1 + w2 * h2;
 }
 
 // wenoExpthreconstruction: the node-independent parts (indicators, tau) once per stencil
+template <bool FAST = false>
 struct WenoExp7 {
     const double *s;
     double dx4, a0, a1, a2, l_lo, l_hi;
     WFV_DEV WenoExp7(const double *st, double dx4_) : s(st), dx4(dx4_) {
         double n[5];
-        wexp_norm(s, n);
+        wexp_norm<FAST>(s, n);
         const double theta = 1;
         const double b0 = theta * 1.0 / 2.0 * fabs(n[0] - 4 * n[1] + 3 * n[2]) + fabs(n[0] - 2 * n[1] + n[2]);
         const double b1 = theta * 1.0 / 2.0 * fabs(n[1] - n[3]) + fabs(n[1] - 2 * n[2] + n[3]);
         const double b2 = theta * 1.0 / 2.0 * fabs(3 * n[2] - 4 * n[3] + n[4]) + fabs(n[2] - 2 * n[3] + n[4]);
         const double tau = n[0] - 4.0 * n[1] + 6.0 * n[2] - 4.0 * n[3] + n[4];
         const double eps = 1e-16;
-        a0 = 1.0 + 2 * (tau * tau) / ((b0 * b0) + eps);
-        a1 = 1.0 + 2 * (tau * tau) / ((b1 * b1) + eps);
-        a2 = 1.0 + 2 * (tau * tau) / ((b2 * b2) + eps);
-        l_lo = wexp_l2h2(s, dx4, true);
-        l_hi = wexp_l2h2(s + 1, dx4, true);
+        if constexpr (FAST) {
+            const double t2 = 2 * (tau * tau);
+            a0 = 1.0 + t2 * rcp_nr((b0 * b0) + eps);
+            a1 = 1.0 + t2 * rcp_nr((b1 * b1) + eps);
+            a2 = 1.0 + t2 * rcp_nr((b2 * b2) + eps);
+        } else {
+            a0 = 1.0 + 2 * (tau * tau) / ((b0 * b0) + eps);
+            a1 = 1.0 + 2 * (tau * tau) / ((b1 * b1) + eps);
+            a2 = 1.0 + 2 * (tau * tau) / ((b2 * b2) + eps);
+        }
+        l_lo = wexp_l2h2<FAST>(s, dx4, true);
+        l_hi = wexp_l2h2<FAST>(s + 1, dx4, true);
     }
     template <int GP>
     WFV_DEV double node() const {
         double d0, d1, d2;
-        wexp_d<GP>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
-        return wexp_combine<GP>(s, d0, d1, d2, a0, a1, a2);
+        wexp_d<GP, FAST>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
+        return wexp_combine<GP, FAST>(s, d0, d1, d2, a0, a1, a2);
     }
 };
 
 // wenoExpWithThincReconstruction: a THINC profile in cells with 1e-6 < u < 1-1e-6, else WENO-Z
 // indicators (eps 1e-16) with the unclamped WENO-EXP linear weights
+template <bool FAST = false>
 struct ExpThinc7 {
     const double *s;
     bool thinc, flat;
@@ -669,10 +694,10 @@ struct ExpThinc7 {
     WFV_DEV ExpThinc7(const double *st, double dx4) : s(st) {
         const double um = s[2], uc = s[3], up = s[4];
         thinc = (1e-6 < uc) & (uc < 1 - 1e-6);
-        l_lo = wexp_l2h2(s, dx4, false);
-        l_hi = wexp_l2h2(s + 1, dx4, false);
+        l_lo = wexp_l2h2<FAST>(s, dx4, false);
+        l_hi = wexp_l2h2<FAST>(s + 1, dx4, false);
         double n[5];
-        wexp_norm(s, n);
+        wexp_norm<FAST>(s, n);
         double t;
         t = n[0] - 4 * n[1] + 3 * n[2];
         double b0 = 0.25 * (t * t);
@@ -688,9 +713,15 @@ struct ExpThinc7 {
         b2 = b2 + 13.0 / 12.0 * (t * t);
         const double eps = 1.0e-16;
         const double tau = fabs(b0 - b2);
-        a0 = 1 + tau / (b0 + eps);
-        a1 = 1 + tau / (b1 + eps);
-        a2 = 1 + tau / (b2 + eps);
+        if constexpr (FAST) {
+            a0 = 1 + tau * rcp_nr(b0 + eps);
+            a1 = 1 + tau * rcp_nr(b1 + eps);
+            a2 = 1 + tau * rcp_nr(b2 + eps);
+        } else {
+            a0 = 1 + tau / (b0 + eps);
+            a1 = 1 + tau / (b1 + eps);
+            a2 = 1 + tau / (b2 + eps);
+        }
         // THINC (include/Utility.hpp:633-661); exp(+-beta) as glibc rounds them
         qmin = stdMin(um, up);
         qmax = stdMax(um, up) - qmin;
@@ -713,8 +744,8 @@ struct ExpThinc7 {
             return 0.5 * qmax * (1 + theta * tanh(beta * ((glp[GP] + 1) / 2 - xcc))) + qmin;
         }
         double d0, d1, d2;
-        wexp_d<GP>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
-        return wexp_combine<GP>(s, d0, d1, d2, a0, a1, a2);
+        wexp_d<GP, FAST>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
+        return wexp_combine<GP, FAST>(s, d0, d1, d2, a0, a1, a2);
     }
 };
 
