@@ -795,7 +795,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
                 wave_sync();
                 dyr = A.dyC[r];
-                if constexpr (!EX) idyr = 1.0 / dyr;
+                if constexpr (!EX) idyr = rcp_nr(dyr);
                 if (xface_lane) {
                     // the neighbour's west face values, as exact FP64 states (base 0)
                     FS nb;  // the neighbour's west face values; momenta relative to its node 0
@@ -832,7 +832,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             if (it >= 2 && owned && live) {
                 const int rr = r - 1;
                 const double dyp = A.dyC[rr];
-                const double idyp = EX ? 0.0 : 1.0 / dyp;
+                const double idyp = EX ? 0.0 : rcp_nr(dyp);
                 const double *Ucur = win[R - 1];
                 double Ub[4];
                 if (A.Un) {
@@ -864,7 +864,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                         tmp = A.cfl / (ex / A.dX + ey / A.dY);
                     } else {
                         const double irho = rcp_nr(Unew[0] + Unew[1]);
-                        tmp = A.cfl / ((fabs(Unew[2] * irho) + e.c) * idX + (fabs(Unew[3] * irho) + e.c) * idY);
+                        tmp = A.cfl * rcp_nr((fabs(Unew[2] * irho) + e.c) * idX + (fabs(Unew[3] * irho) + e.c) * idY);
                     }
                     if (isnan(tmp)) err |= WFV_FLAG_NAN_DT;
                     else dtloc = fmin(dtloc, tmp);
