@@ -233,8 +233,7 @@ WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceS
                 ul[v] = a.get(v, 0, ba[v]);
                 ur[v] = b.get(v, 0, bb[v]);
             }
-            double pref;
-            riemann<double, FLUX, NX, NY>(P, ul, ur, fl, pref);
+            riemann<FLUX, NX, NY>(P, ul, ur, fl);
             const double qw = GLW[g] * width / 2.0;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {

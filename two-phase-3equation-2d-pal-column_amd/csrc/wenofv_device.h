@@ -7,7 +7,9 @@
 //   PrecMixed : WENO smoothness indicators, nonlinear weights and flux arithmetic in
 //               float with hardware reciprocals; the equation of state (volume
 //               fraction, pressure, sound speed), the limiter's theta and the RK
-//               accumulation stay in double.
+//               accumulation stay in double.  The forms that must reconstruct in double
+//               (characteristic, WENO-EXP, primitive3) do so with Newton-refined
+//               reciprocals in place of IEEE divisions.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -41,29 +43,9 @@ struct Phys {
 constexpr double GLW[4] = {0.166666666666666666666666666667, 0.833333333333333333333333333333,
                                                0.833333333333333333333333333333, 0.166666666666666666666666666667};
 
-template <class T>
-WFV_DEV T dv(T a, T b) {  // division: IEEE in double; hardware reciprocal in float
-    if constexpr (sizeof(T) == 8) {
-        return a / b;
-    } else {
-        return a * __builtin_amdgcn_rcpf(b);
-    }
-}
 WFV_DEV double stdMin(double a, double b) { return (b < a) ? b : a; }  // std::min
 WFV_DEV double stdMax(double a, double b) { return (a < b) ? b : a; }  // std::max
-WFV_DEV float stdMin(float a, float b) { return (b < a) ? b : a; }
-WFV_DEV float stdMax(float a, float b) { return (a < b) ? b : a; }
 WFV_DEV double scMin(double a, double b) { return a < b ? a : b; }      // sc_math::Min
-
-// min/max/sqrt of a policy's arithmetic type: std::min/std::max and IEEE sqrt in double; the
-// single hardware instructions in float (they differ only for NaN operands and in the last
-// bit of the square root)
-WFV_DEV double wMin(double a, double b) { return stdMin(a, b); }
-WFV_DEV double wMax(double a, double b) { return stdMax(a, b); }
-WFV_DEV float wMin(float a, float b) { return __builtin_fminf(a, b); }
-WFV_DEV float wMax(float a, float b) { return __builtin_fmaxf(a, b); }
-WFV_DEV double wSqrt(double x) { return sqrt(x); }
-WFV_DEV float wSqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // ------------------------------------------------------------------ equation of state
 // TwoPhaseEquation::getVolumeFrac / getPressure / getSoundSpeed (include/Equations.hpp:84-146)
@@ -158,94 +140,80 @@ WFV_DEV void phy_flux(const T U[4], T rho, T u, T v, T p, T F[4]) {
     }
 }
 
-// In the float policy the fluxes are returned minus pref * n with pref the left state's
-// pressure: the O(1) pressure term of the momentum flux would otherwise dominate its FP32
-// rounding, while its constant part cancels exactly in the divergence once added back in
-// double (the caller adds pref * n).  State jumps are taken in double before rounding.
-// The double policy keeps the reference's formulas unchanged (pref = 0).
-
-template <class T, int NX, int NY>
-WFV_DEV void riemann_llf(const Phys &P, const double dl[4], const double dr[4], T F[4], double &pref) {
+// The all-FP64 policy's Riemann solvers: the reference's formulas, operation by operation.
+template <int NX, int NY>
+WFV_DEV void riemann_llf(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
     // getLLFRiemannFlux (include/Equations.hpp:363-378)
-    constexpr bool EXACT = sizeof(T) == 8;
-    const T UL[4] = {T(dl[0]), T(dl[1]), T(dl[2]), T(dl[3])};
-    const T UR[4] = {T(dr[0]), T(dr[1]), T(dr[2]), T(dr[3])};
     Eos el, er;
-    eos_p<!EXACT>(P, dl, el);
-    eos_p<!EXACT>(P, dr, er);
-    pref = EXACT ? 0.0 : el.p;
-    const T nx = T(NX), ny = T(NY);
-    const T rl = T(el.rho), rr = T(er.rho);
-    const T ul = dv(UL[2], rl), vl = dv(UL[3], rl);
-    const T ur = dv(UR[2], rr), vr = dv(UR[3], rr);
-    const T wsL = fabs(ul * nx + vl * ny) + T(el.c);
-    const T wsR = fabs(ur * nx + vr * ny) + T(er.c);
-    const T ws = wMax(wsL, wsR);
-    T FL[4], FR[4];
-    phy_flux<T, NX, NY>(UL, rl, ul, vl, T(el.p - pref), FL);
-    phy_flux<T, NX, NY>(UR, rr, ur, vr, T(er.p - pref), FR);
+    eos(P, UL, el);
+    eos(P, UR, er);
+    const double nx = NX, ny = NY;
+    const double rl = el.rho, rr = er.rho;
+    const double ul = UL[2] / rl, vl = UL[3] / rl;
+    const double ur = UR[2] / rr, vr = UR[3] / rr;
+    const double wsL = fabs(ul * nx + vl * ny) + el.c;
+    const double wsR = fabs(ur * nx + vr * ny) + er.c;
+    const double ws = stdMax(wsL, wsR);
+    double FL[4], FR[4];
+    phy_flux<double, NX, NY>(UL, rl, ul, vl, el.p, FL);
+    phy_flux<double, NX, NY>(UR, rr, ur, vr, er.p, FR);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        if constexpr (EXACT) F[r] = T(0.5) * (FL[r] + FR[r] - ws * (UR[r] - UL[r]));
-        else F[r] = T(0.5) * (FL[r] + FR[r] - ws * T(dr[r] - dl[r]));
-    }
+    for (int r = 0; r < 4; ++r) F[r] = 0.5 * (FL[r] + FR[r] - ws * (UR[r] - UL[r]));
 }
 
-template <class T, int NX, int NY>
-WFV_DEV void riemann_hllc(const Phys &P, const double dl[4], const double dr[4], T F[4], double &pref) {
+template <int NX, int NY>
+WFV_DEV void riemann_hllc(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
     // getHLLCRiemannFlux (include/Equations.hpp:381-483)
-    constexpr bool EXACT = sizeof(T) == 8;
-    const T UL[4] = {T(dl[0]), T(dl[1]), T(dl[2]), T(dl[3])};
-    const T UR[4] = {T(dr[0]), T(dr[1]), T(dr[2]), T(dr[3])};
     Eos el, er;
-    eos_p<!EXACT>(P, dl, el);
-    eos_p<!EXACT>(P, dr, er);
-    pref = EXACT ? 0.0 : el.p;
-    const T nx = T(NX), ny = T(NY);
-    const T rho_L = T(el.rho), rho_R = T(er.rho);
-    const T u_L = dv(UL[2], rho_L), v_L = dv(UL[3], rho_L);
-    const T u_R = dv(UR[2], rho_R), v_R = dv(UR[3], rho_R);
-    const T pre_L = T(el.p - pref), pre_R = T(er.p - pref), c_L = T(el.c), c_R = T(er.c);
-    const T vn_L = u_L * nx + v_L * ny;
-    const T vn_R = u_R * nx + v_R * ny;
-    T FL[4], FR[4];
+    eos(P, UL, el);
+    eos(P, UR, er);
+    const double nx = NX, ny = NY;
+    const double rho_L = el.rho, rho_R = er.rho;
+    const double u_L = UL[2] / rho_L, v_L = UL[3] / rho_L;
+    const double u_R = UR[2] / rho_R, v_R = UR[3] / rho_R;
+    const double pre_L = el.p, pre_R = er.p, c_L = el.c, c_R = er.c;
+    const double vn_L = u_L * nx + v_L * ny;
+    const double vn_R = u_R * nx + v_R * ny;
+    double FL[4], FR[4];
     FL[0] = UL[0] * vn_L; FL[1] = UL[1] * vn_L;
     FL[2] = UL[2] * vn_L + pre_L * nx; FL[3] = UL[3] * vn_L + pre_L * ny;
     FR[0] = UR[0] * vn_R; FR[1] = UR[1] * vn_R;
     FR[2] = UR[2] * vn_R + pre_R * nx; FR[3] = UR[3] * vn_R + pre_R * ny;
-    const T sL = wSqrt(rho_L), sR = wSqrt(rho_R);
-    const T vn_roe = dv(vn_L * sL + vn_R * sR, sL + sR);
-    T c2_roe = dv(c_L * c_L * sL + c_R * c_R * sR, sL + sR);
-    c2_roe += dv(dv(T(0.5) * sL * sR, sL + sR), sL + sR) * (vn_R - vn_L) * (vn_R - vn_L);
-    const T c_roe = wSqrt(c2_roe);
-    const T S_L = wMin(vn_L - c_L, vn_roe - c_roe);
-    const T S_R = wMax(vn_R + c_R, vn_roe + c_roe);
-    // with pref = p_L the shifted pressures already form the jump pre_R - pre_L
-    const T S_star = dv((S_L - vn_L) * rho_L * vn_L - (S_R - vn_R) * rho_R * vn_R + pre_R - pre_L,
-                        (S_L - vn_L) * rho_L - (S_R - vn_R) * rho_R);
+    const double sL = sqrt(rho_L), sR = sqrt(rho_R);
+    const double vn_roe = (vn_L * sL + vn_R * sR) / (sL + sR);
+    double c2_roe = (c_L * c_L * sL + c_R * c_R * sR) / (sL + sR);
+    c2_roe += 0.5 * sL * sR / (sL + sR) / (sL + sR) * (vn_R - vn_L) * (vn_R - vn_L);
+    const double c_roe = sqrt(c2_roe);
+    const double S_L = stdMin(vn_L - c_L, vn_roe - c_roe);
+    const double S_R = stdMax(vn_R + c_R, vn_roe + c_roe);
+    const double S_star = ((S_L - vn_L) * rho_L * vn_L - (S_R - vn_R) * rho_R * vn_R + pre_R - pre_L) /
+                          ((S_L - vn_L) * rho_L - (S_R - vn_R) * rho_R);
     // The reference's four cases (S* >= 0: F_L or the left star flux; S* < 0: F_R or the
     // right star flux) as one star-flux evaluation on the upwind side's values: the taken
     // case performs the same operations, and a wavefront whose lanes take different cases
     // evaluates one star state instead of two.
-    const bool left = S_star >= T(0);
-    const bool outside = left ? (S_L >= T(0)) : (S_R <= T(0));
-    const T S_K = left ? S_L : S_R, vn_K = left ? vn_L : vn_R, rho_K = left ? rho_L : rho_R;
-    const T u_K = left ? u_L : u_R, v_K = left ? v_L : v_R;
-    T U_K[4], F_K[4];
+    const bool left = S_star >= 0.0;
+    const bool outside = left ? (S_L >= 0.0) : (S_R <= 0.0);
+    const double S_K = left ? S_L : S_R, vn_K = left ? vn_L : vn_R, rho_K = left ? rho_L : rho_R;
+    const double u_K = left ? u_L : u_R, v_K = left ? v_L : v_R;
+    double U_K[4], F_K[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         U_K[r] = left ? UL[r] : UR[r];
         F_K[r] = left ? FL[r] : FR[r];
     }
-    const T k = dv(S_K - vn_K, S_K - S_star);
-    const T Us[4] = {U_K[0], U_K[1], rho_K * (u_K + (S_star - vn_K) * nx), rho_K * (v_K + (S_star - vn_K) * ny)};
+    const double k = (S_K - vn_K) / (S_K - S_star);
+    const double Us[4] = {U_K[0], U_K[1], rho_K * (u_K + (S_star - vn_K) * nx), rho_K * (v_K + (S_star - vn_K) * ny)};
 #pragma unroll
     for (int r = 0; r < 4; ++r) F[r] = outside ? F_K[r] : S_K * (k * Us[r] - U_K[r]) + F_K[r];
 }
 
 // Mixed-policy node solvers on a face whose normal is x (NX = 1) or y.  Each side arrives as
 // its partial densities in double (they alone enter the equation of state) and its normal
-// and tangential momenta in float.  The same algorithms as above, with the products by the
+// and tangential momenta in float.  The fluxes are returned minus pref * n with pref the left
+// state's pressure: the O(1) pressure term of the momentum flux would otherwise dominate its
+// FP32 rounding, while its constant part cancels in the divergence once added back in double
+// (the caller adds pref * n).  The same algorithms as above, with the products by the
 // normal's 0/1 components dropped, the left pressure offset (p_L - pref = 0) known, and the
 // star-state momenta rho_K (u_K + (S* - vn_K)) = rho_K S* and rho_K v_K = m_tK taken in their
 // reduced forms; only rounding differs.
@@ -307,10 +275,10 @@ WFV_DEV void llf_f(const Phys &P, const SideF &L, const SideF &R, const float ju
 }
 
 // getRiemannFlux dispatch (include/Equations.hpp:329-360), resolved at compile time
-template <class T, int FLUX, int NX, int NY>
-WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], T F[4], double &pref) {
-    if constexpr (FLUX == 1) riemann_hllc<T, NX, NY>(P, UL, UR, F, pref);
-    else riemann_llf<T, NX, NY>(P, UL, UR, F, pref);
+template <int FLUX, int NX, int NY>
+WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
+    if constexpr (FLUX == 1) riemann_hllc<NX, NY>(P, UL, UR, F);
+    else riemann_llf<NX, NY>(P, UL, UR, F);
 }
 
 // ------------------------------------------------------------------ eigenvectors
@@ -463,8 +431,8 @@ struct Weno5 {
         } else {
             // WENO5Zthreconstruction (include/Utility.hpp:261-280): beta of the min-max
             // normalised stencil, tau = |beta0 - beta2|, eps = 1e-10
-            const T mx = wMax(wMax(wMax(a, b), wMax(e, d)), c);
-            const T mn = wMin(wMin(wMin(a, b), wMin(e, d)), c);
+            const T mx = stdMax(stdMax(stdMax(a, b), stdMax(e, d)), c);
+            const T mn = stdMin(stdMin(stdMin(a, b), stdMin(e, d)), c);
             const T den = mx - mn + T(1e-20);
             const T a_ = (a - mn) / den, b_ = (b - mn) / den, c_ = (c - mn) / den;
             const T d_ = (d - mn) / den, e_ = (e - mn) / den;
