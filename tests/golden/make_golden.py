@@ -165,7 +165,7 @@ def make_point(rng):
     print("point fixtures written")
 
 
-def run_ref(args, tmp, state=None, nx=None, ny=None):
+def run_ref(args, tmp, state=None, nx=None, ny=None, timeout=120):
     """Run refsolver under 4 MPI ranks in a scratch directory; returns the output prefix."""
     state_path = "-"
     if state is not None:
@@ -177,7 +177,7 @@ def run_ref(args, tmp, state=None, nx=None, ny=None):
     # a reference run that meets a NaN pauses on std::cin and its peers spin in MPI:
     # bound every run and treat a timeout as a failed combination
     res = subprocess.run(cmd, cwd=tmp, stdin=subprocess.DEVNULL, capture_output=True, text=True,
-                         timeout=120)
+                         timeout=timeout)
     if res.returncode != 0:
         raise RuntimeError(res.stdout[-2000:] + res.stderr[-2000:])
     return prefix, res.stdout
@@ -427,6 +427,35 @@ def make_outputs():
     print("output fixtures written")
 
 
+def make_c1():
+    """The tanh-interface advection case in full (128x128 periodic, WENO-JS + LLF, no limiter,
+    RK3, CFL 0.4, to t = 1: the reference's own CPU-runnable configuration) on the reference's
+    4-rank solver: its initial state, every dt and a SHA-256 of its final state (SoA
+    [4][ny][nx] little-endian doubles), plus the reference's loop time on this container's
+    cores (4 MPI ranks)."""
+    import hashlib
+    sys.path.insert(0, os.path.join(REPO, "two-phase-3equation-2d-pal-column_amd"))
+    import cases
+    wl = cases.tanh_interface()
+    p = wl.params
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix, so = run_ref(["run", 0, wl.nx, wl.ny, 2, 0.4, 0, 0, 0, 0, 1.0, p["rho10"], p["rho20"], p["sound1"],
+                              p["sound2"], p["preb"]], tmp, state=wl.U0, timeout=3600)
+        fin = load(prefix, "_final.bin")
+        dts = load(prefix, "_dts.bin")
+        init = load(prefix, "_init.bin")
+    assert np.array_equal(init, wl.U0.ravel())
+    import re
+    m = re.search(r"steps (\d+) t (\S+) loop_seconds (\S+)", so)
+    steps, t, secs = int(m.group(1)), float(m.group(2)), float(m.group(3))
+    np.savez_compressed(os.path.join(HERE, "c1_reference.npz"), U0=wl.U0, dts=dts,
+                        final_sha256=np.array(hashlib.sha256(fin.astype("<f8").tobytes()).hexdigest()),
+                        final_l1=np.abs(fin.reshape(4, -1)).sum(axis=1), t_final=np.array(t),
+                        loop_seconds=np.array(secs), steps=np.array(steps))
+    print(f"c1: {steps} steps to t={t!r}, reference loop {secs:.1f} s = "
+          f"{wl.nx * wl.ny * 3 * steps / secs / 1e6:.3f} M cell-stage/s (4 MPI ranks)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", default=None)
@@ -440,7 +469,8 @@ def main():
             "pointexp": lambda: make_point_exp(np.random.default_rng(7007)),
             "rhsexp": lambda: make_rhs_exp(np.random.default_rng(7008)),
             "runsexp": make_runs_exp,
-            "outputs": make_outputs}
+            "outputs": make_outputs,
+            "c1": make_c1}
     for k, f in jobs.items():
         if a.only is None or k in a.only:
             f()
