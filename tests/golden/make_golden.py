@@ -456,6 +456,31 @@ def make_c1():
           f"{wl.nx * wl.ny * 3 * steps / secs / 1e6:.3f} M cell-stage/s (4 MPI ranks)")
 
 
+def make_settings():
+    """The reference's shipped config/settings.cfg in full (TESTCASE 0 SMOOTH, 100x100, RK3,
+    CFL 0.5, WENO-Z, characteristic, LLF, no limiter) from the case's own initial state to its
+    final time, on the reference's 4-rank solver: every dt, SHA-256 and L1 norms of the final
+    state, loop time."""
+    import hashlib
+    import re
+    sys.path.insert(0, os.path.join(REPO, "two-phase-3equation-2d-pal-column_amd"))
+    import wenofv
+    kw, t_end = wenofv.reference_case(0)
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix, so = run_ref(["run", 0, 100, 100, 2, 0.5, 1, 0, 0, 1, t_end], tmp, timeout=3600)
+        fin = load(prefix, "_final.bin")
+        dts = load(prefix, "_dts.bin")
+        init = load(prefix, "_init.bin")
+    m = re.search(r"steps (\d+) t (\S+) loop_seconds (\S+)", so)
+    steps, t, secs = int(m.group(1)), float(m.group(2)), float(m.group(3))
+    np.savez_compressed(os.path.join(HERE, "settings_reference.npz"), U0=init.reshape(4, 100, 100), dts=dts,
+                        final_sha256=np.array(hashlib.sha256(fin.astype("<f8").tobytes()).hexdigest()),
+                        final_l1=np.abs(fin.reshape(4, -1)).sum(axis=1), t_final=np.array(t),
+                        t_end=np.array(t_end), loop_seconds=np.array(secs), steps=np.array(steps))
+    print(f"settings: {steps} steps to t={t!r}, reference loop {secs:.1f} s = "
+          f"{100 * 100 * 3 * steps / secs / 1e6:.3f} M cell-stage/s (4 MPI ranks)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", default=None)
@@ -470,7 +495,8 @@ def main():
             "rhsexp": lambda: make_rhs_exp(np.random.default_rng(7008)),
             "runsexp": make_runs_exp,
             "outputs": make_outputs,
-            "c1": make_c1}
+            "c1": make_c1,
+            "settings": make_settings}
     for k, f in jobs.items():
         if a.only is None or k in a.only:
             f()
