@@ -320,15 +320,16 @@ def run_single(a, torch, wenofv, cases):
     steps = a.steps or wl.steps or 0
     stream = torch.cuda.Stream()
 
-    def run(precision):
+    def run(precision, U0=None):
+        U0 = wl.U0 if U0 is None else U0
         sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(rows_per_strip=a.ty, precision=precision,
                                                              geometry_ranks=getattr(wl, "geometry_ranks", 1)))
         sol.set_stream(stream.cuda_stream)
-        sol.set_state(wl.U0)
+        sol.set_state(U0)
         tiles = sol.autotune(trials=2, cache=a.tune_cache) if not a.ty else {"rows_per_strip": a.ty}
         if a.warmup:
             sol.advance(fixed_steps=max(a.warmup, 32))  # also captures the step graph
-        sol.set_state(wl.U0)
+        sol.set_state(U0)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -373,9 +374,18 @@ def run_single(a, torch, wenofv, cases):
         gate = ("z1", "rho", "u", "v")
         out["fp64_gpu"] = {"value": cells * 3 * n64 / (ms64 * 1e-3), "ms_per_step": ms64 / n64, "flags": f64}
         out["speedup_vs_fp64_gpu"] = out["value"] / out["fp64_gpu"]["value"]
-        out["accuracy"] = {"rel_l1_per_field": errs, "rel_linf": linf, "max_rel_l1": max(errs[k] for k in gate),
-                           "gate_fields": list(gate), "tolerance": 1e-5,
-                           "within_tolerance": max(errs[k] for k in gate) <= 1e-5}
+        # the FP64 solver's own sensitivity: the same run from the initial state perturbed by
+        # 1e-13 relative (seeded); where it exceeds the tolerance the case is ill-conditioned
+        # in FP64 itself and the mixed error is compared with 10x that spread
+        rng = np.random.default_rng(13)
+        *_, Up, _, _ = run(0, wl.U0 * (1 + 1e-13 * rng.standard_normal(wl.U0.shape)))
+        spread, _, _ = field_errors(Up, U64, wl.params)
+        worst = max(errs[k] for k in gate)
+        spread_max = max(spread[k] for k in gate)
+        out["accuracy"] = {"rel_l1_per_field": errs, "rel_linf": linf, "max_rel_l1": worst,
+                           "gate_fields": list(gate), "tolerance": 1e-5, "within_tolerance": worst <= 1e-5,
+                           "fp64_self_spread_1e-13": spread_max,
+                           "within_tolerance_or_10x_fp64_spread": worst <= max(1e-5, 10 * spread_max)}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_oracle(wl.nx, wl.ny, wl.domain, wl.bc, wl.params, wl.opts, wl.U0,
                                                   a.cpu_seconds, wl.name)

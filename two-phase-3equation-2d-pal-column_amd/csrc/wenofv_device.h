@@ -696,10 +696,20 @@ struct ExpThinc7 {
         theta = (up - uc >= 0) ? 1.0 : -1.0;  // sc_math::Sign (third_party/scTools/MyMath.h:47-50)
         const double beta = 1.5;
         const double exp_beta = 0x1.1ed3fe64fc541p+2, exp_mbeta = 0x1.c8f87724b5c1dp-3;
-        const double C = (uc - qmin + 1e-20) / (qmax + 1e-20);
-        const double B = exp(theta * beta * (2.0 * C - 1.0));
-        const double xc = (exp_beta - B) / (B - exp_mbeta);
-        xcc = log(xc) / (2.0 * beta);
+        if constexpr (FAST) {
+            // mixed policy: the profile's exponential and logarithm in float (hardware
+            // exp2/log2; the node values are volume fractions in [0, 1], where their ~1e-7
+            // absolute error is float rounding)
+            const double C = (uc - qmin + 1e-20) * rcp_nr(qmax + 1e-20);
+            const float B = __expf(float(theta * beta * (2.0 * C - 1.0)));
+            const float xc = (float(exp_beta) - B) * __builtin_amdgcn_rcpf(B - float(exp_mbeta));
+            xcc = double(__logf(xc) * float(1.0 / (2.0 * beta)));
+        } else {
+            const double C = (uc - qmin + 1e-20) / (qmax + 1e-20);
+            const double B = exp(theta * beta * (2.0 * C - 1.0));
+            const double xc = (exp_beta - B) / (B - exp_mbeta);
+            xcc = log(xc) / (2.0 * beta);
+        }
         flat = (uc - um) * (up - uc) < 0;
     }
     template <int GP>
@@ -709,6 +719,12 @@ struct ExpThinc7 {
             constexpr double glp[4] = {-1, -0.447213595499957939281834733746, 0.447213595499957939281834733746, 1};
             if (flat) return s[3];
             const double beta = 1.5;
+            if constexpr (FAST) {
+                // tanh x = 1 - 2 / (exp(2x) + 1), in float (exact limits +-1 at +-inf)
+                const float x = float(beta * ((glp[GP] + 1) / 2 - xcc));
+                const float th = 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * x) + 1.0f);
+                return 0.5 * qmax * (1 + theta * double(th)) + qmin;
+            }
             return 0.5 * qmax * (1 + theta * tanh(beta * ((glp[GP] + 1) / 2 - xcc))) + qmin;
         }
         double d0, d1, d2;
