@@ -471,7 +471,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     W u4[4] = {sv(t, 0), sv(t, 1), sv(t, 2), sv(t, 3)};
                     matvec(Lm, u4, pc[t]);
                 }
-#pragma unroll 1
+#pragma unroll VU
                 for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
                     W s5[5] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]};
                     W o[4];
@@ -601,7 +601,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                         matvec(Lm, y4, pc[t]);
                     }
                     W uc[4][4];  // [gx][k]
-#pragma unroll 1
+#pragma unroll VU
                     for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
                         WenoSel<EX, W, SCHEME> st(pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]);
                         rotate_push(uc[0], st.template node<0>());
