@@ -436,7 +436,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     double u4[4] = {sv(t, 0), sv(t, 1), sv(t, 2), sv(t, 3)};
                     matvec(Lm, u4, pc[t]);
                 }
-#pragma unroll 1
+#pragma unroll VU
                 for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
                     const double s7[7] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0], pc[5][0], pc[6][0]};
                     double o[4];
@@ -453,7 +453,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     for (int v = 0; v < 4; ++v) L.y[g * 4 + v][lane] = yv[v];
                 }
             } else if constexpr (R == 3) {
-#pragma unroll 1
+#pragma unroll VU
                 for (int v = 0; v < 4; ++v) {
                     double s7[7];
 #pragma unroll
@@ -538,7 +538,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             if constexpr (R == 3 && RECON == 1) {
                 double Lm[16], Rm[16];
                 eigen_LR<double, true, !EX>(P, Uc, Lm, Rm);
-#pragma unroll 1
+#pragma unroll VU
                 for (int gy = 0; gy < 4; ++gy) {  // node row gy sits in slot 0 of ug[.][0]
                     double pc[7][4];
 #pragma unroll
@@ -576,7 +576,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                             ug[gx][b][a] = t;
                         }
             } else if constexpr (R == 3) {
-#pragma unroll 1
+#pragma unroll VU
                 for (int v = 0; v < 4; ++v) {  // variable v sits in slot 3 after the loop
 #pragma unroll
                     for (int gy = 0; gy < 4; ++gy) {
