@@ -333,10 +333,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr int R = ST::R, NW = ST::NW, HALO = ST::HALO, SW = ST::SW;
     constexpr bool EX = PR::exact;
     constexpr bool DEV = !EX && R == 2;
-    // Loops over the 4 variables: unrolled in the mixed policy (no register rotation, no
-    // spills, about 14 % fewer instructions per cell), rolled in the all-FP64 policy, whose
-    // unrolled code would outgrow the 64 KiB instruction cache.
-    constexpr int VU = EX ? 1 : 4;
+    // Loops over the 4 variables or characteristic fields: unrolled (no register rotation,
+    // fewer spills; measured faster in every kernel, the all-FP64 ones included, although
+    // several of those then outgrow the 64 KiB instruction cache).
+    constexpr int VU = 4;
     const int lane = threadIdx.x & (LANES - 1);
 
     const StripPlan &sp = A.sp;
