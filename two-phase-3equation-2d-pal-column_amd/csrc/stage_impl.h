@@ -956,10 +956,10 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
     BatchStageArgs B) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
     if constexpr (PERSIST) {
-        for (int item = blockIdx.x; item < B.total_items; item += gridDim.x)
+        for (int item = (int)blockIdx.x; item < B.total_items; item += (int)gridDim.x)
             batch_item<PR, RECON, SCHEME, FLUX>(B, item, L);
     } else {
-        if (blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX>(B, blockIdx.x, L);
+        if ((int)blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX>(B, (int)blockIdx.x, L);
     }
 }
 
