@@ -427,20 +427,20 @@ def make_outputs():
     print("output fixtures written")
 
 
-def make_c1():
+def make_c1(recon=0, name="c1_reference.npz"):
     """The tanh-interface advection case in full (128x128 periodic, WENO-JS + LLF, no limiter,
-    RK3, CFL 0.4, to t = 1: the reference's own CPU-runnable configuration) on the reference's
-    4-rank solver: its initial state, every dt and a SHA-256 of its final state (SoA
-    [4][ny][nx] little-endian doubles), plus the reference's loop time on this container's
-    cores (4 MPI ranks)."""
+    RK3, CFL 0.4, to t = 1: the reference's own CPU-runnable configuration; RECONSTRUCTIONTYPE
+    0, or 1 for the characteristic variant) on the reference's 4-rank solver: its initial
+    state, every dt and a SHA-256 of its final state (SoA [4][ny][nx] little-endian doubles),
+    plus the reference's loop time on this container's cores (4 MPI ranks)."""
     import hashlib
     sys.path.insert(0, os.path.join(REPO, "two-phase-3equation-2d-pal-column_amd"))
     import cases
     wl = cases.tanh_interface()
     p = wl.params
     with tempfile.TemporaryDirectory() as tmp:
-        prefix, so = run_ref(["run", 0, wl.nx, wl.ny, 2, 0.4, 0, 0, 0, 0, 1.0, p["rho10"], p["rho20"], p["sound1"],
-                              p["sound2"], p["preb"]], tmp, state=wl.U0, timeout=3600)
+        prefix, so = run_ref(["run", 0, wl.nx, wl.ny, 2, 0.4, 0, 0, 0, recon, 1.0, p["rho10"], p["rho20"],
+                              p["sound1"], p["sound2"], p["preb"]], tmp, state=wl.U0, timeout=7200)
         fin = load(prefix, "_final.bin")
         dts = load(prefix, "_dts.bin")
         init = load(prefix, "_init.bin")
@@ -448,11 +448,11 @@ def make_c1():
     import re
     m = re.search(r"steps (\d+) t (\S+) loop_seconds (\S+)", so)
     steps, t, secs = int(m.group(1)), float(m.group(2)), float(m.group(3))
-    np.savez_compressed(os.path.join(HERE, "c1_reference.npz"), U0=wl.U0, dts=dts,
+    np.savez_compressed(os.path.join(HERE, name), U0=wl.U0, dts=dts,
                         final_sha256=np.array(hashlib.sha256(fin.astype("<f8").tobytes()).hexdigest()),
                         final_l1=np.abs(fin.reshape(4, -1)).sum(axis=1), t_final=np.array(t),
                         loop_seconds=np.array(secs), steps=np.array(steps))
-    print(f"c1: {steps} steps to t={t!r}, reference loop {secs:.1f} s = "
+    print(f"{name}: {steps} steps to t={t!r}, reference loop {secs:.1f} s = "
           f"{wl.nx * wl.ny * 3 * steps / secs / 1e6:.3f} M cell-stage/s (4 MPI ranks)")
 
 
@@ -496,6 +496,7 @@ def main():
             "runsexp": make_runs_exp,
             "outputs": make_outputs,
             "c1": make_c1,
+            "c1char": lambda: make_c1(1, "c1_char_reference.npz"),
             "settings": make_settings}
     for k, f in jobs.items():
         if a.only is None or k in a.only:

@@ -223,9 +223,9 @@ WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceS
 #pragma unroll
     for (int v = 0; v < 4; ++v) G[v] = 0.0;
     if constexpr (PR::exact) {
-        // node in register slot 0 with quadrature weight GLW[g] width / 2; the node loop is
-        // rolled, the node values rotating through the registers
-#pragma unroll 1
+        // node in register slot 0 with quadrature weight GLW[g] width / 2; the node values
+        // rotate into that slot (unrolled, the rotation costs no instructions)
+#pragma unroll
         for (int g = 0; g < 4; ++g) {
             double ul[4], ur[4], fl[4];
 #pragma unroll
