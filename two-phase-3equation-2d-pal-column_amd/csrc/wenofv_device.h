@@ -215,8 +215,7 @@ WFV_DEV void riemann_hllc(const Phys &P, const double UL[4], const double UR[4],
 // FP32 rounding, while its constant part cancels in the divergence once added back in double
 // (the caller adds pref * n).  The same algorithms as above, with the products by the
 // normal's 0/1 components dropped, the left pressure offset (p_L - pref = 0) known, and the
-// star-state momenta rho_K (u_K + (S* - vn_K)) = rho_K S* and rho_K v_K = m_tK taken in their
-// reduced forms; only rounding differs.
+// HLLC star-state flux taken in its closed form (below); only rounding differs.
 struct SideF {
     double a, b;  // z1 rho1, z2 rho2
     float mn, mt; // normal and tangential momentum
@@ -228,17 +227,18 @@ WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, float F[4], d
     eos_flux(P, R.a, R.b, er);
     pref = el.p;
     const float rho_L = float(el.rho), rho_R = float(er.rho);
-    const float vn_L = L.mn * __builtin_amdgcn_rcpf(rho_L), vn_R = R.mn * __builtin_amdgcn_rcpf(rho_R);
+    const float irho_L = __builtin_amdgcn_rcpf(rho_L);
+    const float vn_L = L.mn * irho_L, vn_R = R.mn * __builtin_amdgcn_rcpf(rho_R);
     const float pre_R = float(er.p - el.p), c_L = float(el.c), c_R = float(er.c);
     const float aL = float(L.a), bL = float(L.b), aR = float(R.a), bR = float(R.b);
-    // fluxes [z1rho1, z2rho2, normal momentum, tangential momentum]
-    const float FL[4] = {aL * vn_L, bL * vn_L, L.mn * vn_L, L.mt * vn_L};
-    const float FR[4] = {aR * vn_R, bR * vn_R, R.mn * vn_R + pre_R, R.mt * vn_R};
-    const float sL = __builtin_amdgcn_sqrtf(rho_L), sR = __builtin_amdgcn_sqrtf(rho_R);
-    const float is = __builtin_amdgcn_rcpf(sL + sR);
-    const float vn_roe = (vn_L * sL + vn_R * sR) * is;
+    // Roe weights sqrt(rho_K) / (sqrt(rho_L) + sqrt(rho_R)) from one square root:
+    // w_L = 1 / (1 + t), w_R = t w_L with t = sqrt(rho_R / rho_L)
+    const float t = __builtin_amdgcn_sqrtf(rho_R * irho_L);
+    const float wL = __builtin_amdgcn_rcpf(1.0f + t), wR = t * wL;
     const float dvn = vn_R - vn_L;
-    const float c2_roe = (c_L * c_L * sL + c_R * c_R * sR) * is + (0.5f * sL * sR * is * is) * (dvn * dvn);
+    const float vn_roe = wR * dvn + vn_L;
+    const float cL2 = c_L * c_L;
+    const float c2_roe = wR * (c_R * c_R - cL2) + cL2 + (0.5f * wL * wR) * (dvn * dvn);
     const float c_roe = __builtin_amdgcn_sqrtf(c2_roe);
     const float S_L = __builtin_fminf(vn_L - c_L, vn_roe - c_roe);
     const float S_R = __builtin_fmaxf(vn_R + c_R, vn_roe + c_roe);
@@ -247,14 +247,20 @@ WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, float F[4], d
     const bool left = S_star >= 0.0f;
     const bool outside = left ? (S_L >= 0.0f) : (S_R <= 0.0f);
     const float S_K = left ? S_L : S_R, vn_K = left ? vn_L : vn_R, rho_K = left ? rho_L : rho_R;
+    const float q_K = left ? qL : qR, p_K = left ? 0.0f : pre_R;
     const float U_K[4] = {left ? aL : aR, left ? bL : bR, left ? L.mn : R.mn, left ? L.mt : R.mt};
+    // F_K + S_K (U*_K - U_K) with U*_K = k (a, b, rho S*, m_t)_K, k = (S_K - vn_K)/(S_K - S*),
+    // in closed form: k (S_K - S*) = S_K - vn_K turns the transported components into
+    // U_K k S* and the normal momentum into (k rho_K S*) S* + p*, where
+    // p* = p_K + q_K (S* - vn_K); outside the fan F_K = U_K vn_K (+ p_K)
     const float k = (S_K - vn_K) * __builtin_amdgcn_rcpf(S_K - S_star);
-    const float Us[4] = {U_K[0], U_K[1], rho_K * S_star, U_K[3]};
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const float FK = left ? FL[r] : FR[r];
-        F[r] = outside ? FK : S_K * (k * Us[r] - U_K[r]) + FK;
-    }
+    const float phi = outside ? vn_K : k * S_star;
+    const float M = outside ? U_K[2] : rho_K * S_star;
+    const float Pn = outside ? p_K : q_K * (S_star - vn_K) + p_K;
+    F[0] = U_K[0] * phi;
+    F[1] = U_K[1] * phi;
+    F[2] = phi * M + Pn;
+    F[3] = U_K[3] * phi;
 }
 
 // jump = right minus left state, formed by the caller from exact double differences of the
@@ -268,10 +274,12 @@ WFV_DEV void llf_f(const Phys &P, const SideF &L, const SideF &R, const float ju
     const float vn_L = L.mn * __builtin_amdgcn_rcpf(rho_L), vn_R = R.mn * __builtin_amdgcn_rcpf(rho_R);
     const float pre_R = float(er.p - el.p);
     const float ws = __builtin_fmaxf(fabsf(vn_L) + float(el.c), fabsf(vn_R) + float(er.c));
-    const float FL[4] = {float(L.a) * vn_L, float(L.b) * vn_L, L.mn * vn_L, L.mt * vn_L};
-    const float FR[4] = {float(R.a) * vn_R, float(R.b) * vn_R, R.mn * vn_R + pre_R, R.mt * vn_R};
+    // 0.5 (F_L + F_R - ws jump) with the halves taken into the speeds
+    const float hL = 0.5f * vn_L, hR = 0.5f * vn_R, hw = 0.5f * ws;
+    const float UL[4] = {float(L.a), float(L.b), L.mn, L.mt}, UR[4] = {float(R.a), float(R.b), R.mn, R.mt};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) F[r] = 0.5f * (FL[r] + FR[r] - ws * jump[r]);
+    for (int r = 0; r < 4; ++r) F[r] = UL[r] * hL + (UR[r] * hR - hw * jump[r]);
+    F[2] += 0.5f * pre_R;
 }
 
 // getRiemannFlux dispatch (include/Equations.hpp:329-360), resolved at compile time
