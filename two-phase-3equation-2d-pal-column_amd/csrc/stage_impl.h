@@ -644,7 +644,8 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                         for (int t = 0; t < 5; ++t) {
                             const int l = lane - 2 + t;
-                            st5[t] = W(L.base[v][l] - bc[0]) + L.y[gy * 4 + v][l];
+                            // the centre's base is bc[0] itself: its value is its y-deviation
+                            st5[t] = t == 2 ? L.y[gy * 4 + v][l] : W(L.base[v][l] - bc[0]) + L.y[gy * 4 + v][l];
                         }
                         WenoSel<EX, W, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
                         rotate_push(ug[0][gy], st.template node<0>());
