@@ -35,6 +35,15 @@ sys.path.insert(0, PKG)
 METRIC = "cell-stage-updates/s mixed+batched vs FP64 GPU at ≤1e-5 L1; achieved HBM GB/s"
 BYTES_PER_CELL_STAGE = 256.0 / 3.0  # FP64 state, 4 fields, SSP-RK3: (64 + 96 + 96) B per cell-step
 HBM_PEAK_GBS = 8000.0
+# the algorithm's floating-point operations per cell-stage (reconstruction, limiter, Riemann
+# fluxes, RK combination and the CFL step; each value counted once; divide = square root =
+# compare = 1), keyed by (scheme, reconstruction, flux, limiter): the operation count of the
+# oracle, which performs the FP64 path's arithmetic (oracle/flop_count.cpp; the figures are
+# checked against it by tests/test_flop_count.py)
+ALGO_FLOPS = {(0, 0, 0, 0): 2781.0, (1, 0, 1, 1): 3868.3, (1, 0, 0, 0): 3259.6, (1, 1, 0, 0): 5313.7,
+              (1, 1, 1, 1): 5902.0, (1, 2, 1, 1): 3965.5, (1, 3, 1, 1): 3981.5, (1, 4, 1, 1): 3629.9,
+              (2, 0, 1, 1): 4377.9}
+VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X, FMA = 2
 
 
 def parse():
@@ -141,6 +150,20 @@ def roofline(cells, stage_ms, traffic=None):
                     "traffic = measured HBM bytes per stage launch (rocprofv3 summary under profiles/)"}
 
 
+def algo_flops(opts, rate):
+    """Algorithmic FLOP rate of a run at `rate` cell-stage updates/s, against the vector peaks."""
+    f = ALGO_FLOPS.get((opts["scheme"], opts["recon"], opts["flux"], opts["bp"]))
+    if f is None:
+        return None
+    tf = f * rate / 1e12
+    return {"per_cell_stage": f, "achieved_tflops": tf,
+            "frac_fp64_vector_peak": tf / VECTOR_PEAK_TFLOPS["fp64"],
+            "frac_fp32_vector_peak": tf / VECTOR_PEAK_TFLOPS["fp32"],
+            "note": "operations of the algorithm (each value once, divide = sqrt = compare = 1; "
+                    "oracle/flop_count.cpp) x cell-stage updates/s; the hardware executes more: an FP64 "
+                    "division or square root is a sequence of ~10 instructions, and packed FP32 is not used"}
+
+
 def plain_fp64_per_instance(torch, wenofv, wls, opts, stream, steps):
     """The plain all-FP64 single-domain solver run on one instance after another (the path a
     user without batching takes): each instance warms up (graph capture), then `steps`
@@ -212,6 +235,7 @@ def run_batch(a, torch, wenofv, cases):
                       "lane_instructions_per_cell_stage": profiled(prof, "lane_instructions_per_cell_stage"),
                       "profile_stage_ms": (profiled(prof, "stage_kernel_avg_ns") or 0) / 1e6 or None,
                       "bench_stage_ms": results["mixed"]["ms"] / (3 * steps), "source": f"profiles/{prof}",
+                      "algorithmic_flops": algo_flops(opts, rate),
                       "note": "busy_frac = SQ_ACTIVE_INST_VALU quad-cycles / (SIMDs x duration x clock / 4): the "
                               "share of VALU time the kernel keeps busy, multi-cycle transcendental and FP64 "
                               "instructions included; issue_frac = SQ_INSTS_VALU per SIMD cycle / 0.25 (a "
@@ -269,6 +293,7 @@ def run_batch(a, torch, wenofv, cases):
 
         out["fp64_batched_gpu"] = {"value": fp64_rate, "ms_per_step": results["fp64"]["ms"] / steps,
                                    "flags": results["fp64"]["flags"],
+                                   "algorithmic_flops": algo_flops(opts, fp64_rate),
                                    "note": "the same batch and launches in all-FP64 (precision is the only change)"}
         out["speedup_vs_fp64_batched_gpu"] = rate / fp64_rate
         out["accuracy"] = {
@@ -369,6 +394,7 @@ def run_single(a, torch, wenofv, cases):
         "config": {"workload": desc, "precision": "fp64" if prec == 0 else "mixed", "cells": cells,
                    "final_time": t, "flags": flags, "finite": bool(np.isfinite(U).all()), "tiles": tiles},
         "roofline": roofline(cells, ms / (3 * n)),
+        "algorithmic_flops": algo_flops(wl.opts, cells * 3 * n / (ms * 1e-3)),
     }
     z1 = derived(U, **wl.params)[0]
     out["config"]["z1_range"] = [float(z1.min()), float(z1.max())]
