@@ -91,3 +91,44 @@ def test_mixed_tanh_interface_periodic():
     err = field_errors(res[1], res[0], wl.params)
     print({k: f"{v:.2e}" for k, v in err.items()}, flush=True)
     assert max(err.values()) <= TOL, err
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_mixed_supersonic_interface(flux):
+    """A tanh interface carried at u = 30, v = -25 past sound speeds of at most 10: every
+    x-face's HLLC solution lies left of the wave fan's left edge (the left flux) and every
+    y-face's right of its right edge (the right flux), branches no subsonic case reaches.
+
+    z1, rho, u and v must agree with FP64 to 1e-5.  The partial densities and momenta must
+    agree to 1e-5 or to the FP64 solver's own response to one float rounding of its initial
+    state (a 6e-8 relative perturbation), whichever is larger: on this case that response
+    reaches 2.5e-5 in z2 rho2 with LLF (measured on the oracle), the floor for any
+    arithmetic that rounds to FP32 anywhere."""
+    prm = dict(rho10=1.0, rho20=0.1, sound1=10.0, sound2=10.0, preb=0.0)
+    dom = (0.0, 1.0, 0.0, 1.0)
+
+    def pt(X, Y):
+        r = np.hypot(X - 0.5, Y - 0.5)
+        z1 = 1e-6 + (1 - 2e-6) * (0.5 + 0.5 * np.tanh((0.2 - r) / 0.02))
+        one = np.ones_like(X)
+        return cases.conservative(z1, one, 30.0 * one, -25.0 * one, **prm)
+
+    U0 = cases.cell_average(64, 64, dom, pt)
+    print(f"mixed supersonic flux{flux}", flush=True)
+    rng = np.random.default_rng(8)
+    out = {}
+    for key, prec, U in ((0, 0, U0), (1, 1, U0), ("p", 0, U0 * (1 + 6e-8 * rng.standard_normal(U0.shape)))):
+        sol = wenofv.Solver(64, 64, domain=dom, bc=0, geometry_ranks=1, scheme=1, recon=0, flux=flux, bp=1, rk=2,
+                            cfl=0.4, precision=prec, **prm)
+        sol.set_state(U)
+        n, t, dts = sol.advance(fixed_steps=40, max_dts=64)
+        out[key] = (sol.get_state(), sol.flags(), t)
+    assert all(o[1] == 0 and o[2] > 0 for o in out.values())
+    err = field_errors(out[1][0], out[0][0], prm)
+    spread = field_errors(out["p"][0], out[0][0], prm)
+    print("mixed", {k: f"{v:.2e}" for k, v in err.items()}, flush=True)
+    print("fp64 6e-8 spread", {k: f"{v:.2e}" for k, v in spread.items()}, flush=True)
+    assert max(err[k] for k in ("z1", "rho", "u", "v")) <= TOL, err
+    assert all(err[k] <= max(TOL, spread[k]) for k in err), (err, spread)
+    z1 = derived(out[1][0], **prm)["z1"]
+    assert z1.min() >= 0.0 and z1.max() <= 1.0
