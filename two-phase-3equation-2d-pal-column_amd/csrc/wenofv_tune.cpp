@@ -38,6 +38,25 @@ std::vector<int> row_candidates(int ny) {
     return c;
 }
 
+// A grid of a few hundred thousand to a few million cells gives fewer wavefronts than the
+// device holds at once, or a last round that is nearly empty: a launch lasts as long as
+// its most loaded SIMD.  Added candidates: for 1 and 2 resident wavefronts per SIMD and 1-4
+// rounds, the fewest rows per strip whose wavefront count fits that many rounds exactly.
+std::vector<int> occupancy_candidates(int nx, int ny, int halo, int cus) {
+    std::vector<int> c;
+    const int top = std::min(ny, 128);
+    for (int per_cu : {4, 8})
+        for (int rounds = 1; rounds <= 4; ++rounds) {
+            const long long cap = (long long)rounds * per_cu * cus;
+            for (int ty = 1; ty <= top; ++ty)
+                if (strip_plan(nx, ny, ty, halo).total <= cap) {
+                    c.push_back(ty);
+                    break;
+                }
+        }
+    return c;
+}
+
 // Best-of-REPEATS mean duration (ms) of `trials` launches issued by `launch` on `s`.
 template <class L>
 hipError_t time_launches(hipStream_t s, int trials, L launch, float *ms) {
@@ -128,7 +147,15 @@ int wfv_autotune(wfv_ctx *c, int trials, wfv_tuning *best) {
     const int start = c->ty;
     int best_ty = start;
     float best_ms = 1e30f;
-    for (int ty : row_candidates(c->ny)) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    std::vector<int> cand = row_candidates(c->ny);
+    for (int ty : occupancy_candidates(c->nx, c->ny, strip_halo(c->opt.scheme, c->opt.reconstruction), cus))
+        cand.push_back(ty);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    for (int ty : cand) {
         c->ty = ty;
         float ms;
         hipError_t e = time_ctx_stage(c, trials, &ms);
