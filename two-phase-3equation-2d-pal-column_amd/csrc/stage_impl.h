@@ -64,7 +64,8 @@ WFV_DEV int map_index(int i, int n, int bc, bool &mirrored) {
 // A state across mirrors: slip walls negate the momentum normal to each mirror (mu: x, mv:
 // y); no-slip walls negate both components once per mirror.
 template <class T>
-WFV_DEV void load_cell(const double *__restrict__ U, size_t nc, size_t off, bool mu, bool mv, int bc, T out[4]) {
+WFV_DEV void load_cell(const double *__restrict__ Up, size_t nc, size_t off, bool mu, bool mv, int bc, T out[4]) {
+    const WFV_GLOBAL double *U = (const WFV_GLOBAL double *)Up;
     double a = U[off], b = U[nc + off], c = U[2 * nc + off], d = U[3 * nc + off];
     const bool nu = bc == 2 ? mu != mv : mu;
     const bool nv = bc == 2 ? mu != mv : mv;
@@ -317,8 +318,11 @@ WFV_DEV void recon7_x(const double s[7], int v, double dx4, bool all, double out
     }
 }
 
-// One wavefront's strip of one stage (the whole algorithm; see the file comment).
-template <class PR, int RECON, int SCHEME, int FLUX>
+// One wavefront's strip of one stage (the whole algorithm; see the file comment).  PF:
+// load each row of the register window one row ahead, so that its global-memory latency
+// overlaps a row of arithmetic (for kernels that run one wavefront per SIMD, where no
+// second wavefront hides it).
+template <class PR, int RECON, int SCHEME, int FLUX, bool PF = false>
 WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     // Reconstruction arithmetic.  The characteristic form projects with eigenvector
     // matrices whose condition number reaches ~1e8 in near-pure phases (z1 -> 0 or 1), which
@@ -378,7 +382,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         if (A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny)) {
             const size_t rs = ring_size(nx, ny), o = ghost_offset(nx, ny, xc, row);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) out[v] = A.ghost[v * rs + o];
+            for (int v = 0; v < 4; ++v) out[v] = ((const WFV_GLOBAL double *)A.ghost)[v * rs + o];
             return;
         }
         bool mv;
@@ -403,6 +407,12 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         top_base[v] = 0.0;
     }
 
+    // PF: the row the next iteration takes into the window (the same rows as without PF)
+    double ahead[4];
+    if constexpr (PF) {
+        if (nrow_w >= 0) load_row(y0 + R, ahead);
+    }
+
     for (int it = 0; it <= nrow_w + 1; ++it) {
         const int r = y0 - 1 + it;
         const bool live = it <= nrow + 1;  // this lane's segment still has rows
@@ -414,7 +424,13 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     win[k][v] = win[k + 1][v];
                     if constexpr (RECON >= 2) pwin[k][v] = pwin[k + 1][v];
                 }
-            load_row(r + R, win[NW - 1]);
+            if constexpr (PF) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) win[NW - 1][v] = ahead[v];
+                if (it <= nrow_w) load_row(r + R + 1, ahead);
+            } else {
+                load_row(r + R, win[NW - 1]);
+            }
             if constexpr (RECON >= 2) to_prim<double, RECON, !EX>(P, win[NW - 1], pwin[NW - 1]);
         }
         const double *Uc = win[R];                          // cell (xc, r)
@@ -794,7 +810,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                     for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
                 wave_sync();
-                dyr = A.dyC[r];
+                dyr = ((const WFV_GLOBAL double *)A.dyC)[r];
                 if constexpr (!EX) idyr = rcp_nr(dyr);
                 if (xface_lane) {
                     // the neighbour's west face values, as exact FP64 states (base 0)
@@ -831,7 +847,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             //      1831-1872, 466-471)
             if (it >= 2 && owned && live) {
                 const int rr = r - 1;
-                const double dyp = A.dyC[rr];
+                const double dyp = ((const WFV_GLOBAL double *)A.dyC)[rr];
                 const double idyp = EX ? 0.0 : rcp_nr(dyp);
                 const double *Ucur = win[R - 1];
                 double Ub[4];
@@ -849,7 +865,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     else rhs -= G[v] * idxc * idyp;
                     if (v == 3 && A.gravity != 0.0) rhs -= (Ucur[0] + Ucur[1]) * A.gravity;
                     Unew[v] = A.a * Ub[v] + A.b * Ucur[v] + cdt * rhs;
-                    A.Uo[v * nc + (size_t)rr * nx + xc] = Unew[v];
+                    ((WFV_GLOBAL double *)A.Uo)[v * nc + (size_t)rr * nx + xc] = Unew[v];
                 }
                 if (A.dtmin) {
                     // calculateDeltaT on the new state (src/WENOFV.cpp:489-513)
@@ -902,7 +918,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
-    stage_body<PR, RECON, SCHEME, FLUX>(A, blockIdx.x, L);
+    stage_body<PR, RECON, SCHEME, FLUX, MINW == 1>(A, blockIdx.x, L);
 }
 
 // Batch of independent instances with uneven grids in one persistent launch: the
@@ -912,7 +928,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // items of the flattened (instance, strip) list; each instance keeps its own grid,
 // parameters, RK registers and device-side time step.  PERSIST: a grid smaller than the
 // item list strides over it (a persistent launch); otherwise one wavefront per item.
-template <class PR, int RECON, int SCHEME, int FLUX>
+template <class PR, int RECON, int SCHEME, int FLUX, bool PF>
 WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     const int i = B.item_inst[item];
     const BatchInst &I = B.inst[i];
@@ -949,7 +965,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.faces_out = nullptr;
     A.theta_out = nullptr;
     A.theta_flag = nullptr;
-    stage_body<PR, RECON, SCHEME, FLUX>(A, item - I.item_begin, L);
+    stage_body<PR, RECON, SCHEME, FLUX, PF>(A, item - I.item_begin, L);
 }
 
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST>
@@ -958,9 +974,9 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
     if constexpr (PERSIST) {
         for (int item = (int)blockIdx.x; item < B.total_items; item += (int)gridDim.x)
-            batch_item<PR, RECON, SCHEME, FLUX>(B, item, L);
+            batch_item<PR, RECON, SCHEME, FLUX, MINW == 1>(B, item, L);
     } else {
-        if ((int)blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX>(B, (int)blockIdx.x, L);
+        if ((int)blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX, MINW == 1>(B, (int)blockIdx.x, L);
     }
 }
 

@@ -18,6 +18,10 @@
 #include "weno_constants.h"
 
 #define WFV_DEV __device__ __forceinline__
+// global-memory view of a device pointer: accesses through it compile to global_* rather
+// than flat_* instructions, which wait on the vector-memory counter only (a flat access
+// also holds up the wavefront's next wait for its LDS operations)
+#define WFV_GLOBAL __attribute__((address_space(1)))
 
 namespace wfv {
 
