@@ -236,10 +236,10 @@ def run_batch(a, torch, wenofv, cases):
                       "profile_stage_ms": (profiled(prof, "stage_kernel_avg_ns") or 0) / 1e6 or None,
                       "bench_stage_ms": results["mixed"]["ms"] / (3 * steps), "source": f"profiles/{prof}",
                       "algorithmic_flops": algo_flops(opts, rate),
-                      "note": "busy_frac = SQ_ACTIVE_INST_VALU quad-cycles / (SIMDs x duration x clock / 4): the "
-                              "share of VALU time the kernel keeps busy, multi-cycle transcendental and FP64 "
-                              "instructions included; issue_frac = SQ_INSTS_VALU per SIMD cycle / 0.25 (a "
-                              "wave64 VALU instruction occupies a SIMD for 4 cycles)"}
+                      "note": "issue_frac = SQ_INSTS_VALU per SIMD cycle / 0.25 (a wave64 VALU instruction "
+                              "occupies a SIMD for 4 cycles); busy_frac = SQ_ACTIVE_INST_VALU quad-cycles / "
+                              "(SIMDs x duration x clock / 4), multi-cycle transcendental and FP64 instructions "
+                              "included (a sum over waves: it can pass 1 when two waves' instructions overlap)"}
     if not a.no_fp64:
         plain_rate, plain_ms = plain_fp64_per_instance(torch, wenofv, wls, opts, stream, a.plain_steps)
         out["fp64_plain_gpu"] = {
