@@ -59,3 +59,26 @@ def test_batch_time_driven_per_instance_dt():
         ns, ts, _ = sol.advance(t_end=2e-3)
         assert ns == n[k] and ts == t[k]
         np.testing.assert_array_equal(b.get_state(k), sol.get_state())
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("scheme,recon", [(2, 0), (1, 4)])
+def test_batch_seven_point_forms_equal_single_solvers(precision, scheme, recon):
+    """WENO-EXP and primitive3/THINC (halo 4, 56-column strips) in a batch of uneven
+    grids: every instance equals its single-domain solver bit for bit."""
+    wls = make_instances(21 + recon, 4)
+    opts = dict(scheme=scheme, recon=recon, flux=1, bp=1, rk=2, cfl=0.4, precision=precision)
+    insts = [dict(nx=wl.nx, ny=wl.ny, bc=1 if k % 2 == 0 else 0, domain=wl.domain, **wl.params)
+             for k, wl in enumerate(wls)]
+    print(f"batch 7-point precision{precision} scheme{scheme} recon{recon}", flush=True)
+    b = wenofv.Batch(insts, geometry_ranks=1, **opts)
+    for k, wl in enumerate(wls):
+        b.set_state(k, wl.U0)
+    n, t, f = b.advance(fixed_steps=5)
+    assert (n == 5).all() and (f == 0).all()
+    for k, wl in enumerate(wls):
+        sol = wenofv.Solver(wl.nx, wl.ny, domain=wl.domain, bc=insts[k]["bc"], geometry_ranks=1, **wl.params,
+                            **opts)
+        sol.set_state(wl.U0)
+        sol.advance(fixed_steps=5)
+        np.testing.assert_array_equal(b.get_state(k), sol.get_state())
