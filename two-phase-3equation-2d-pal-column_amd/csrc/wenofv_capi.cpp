@@ -118,7 +118,7 @@ int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active,
 
 extern "C" {
 
-const char *wfv_version(void) { return "wenofv-gfx950 1.0 (fp64, mixed; js, z; cons, char, prim, prim2)"; }
+const char *wfv_version(void) { return "wenofv-gfx950 1.1 (fp64, mixed; js, z, exp; cons, char, prim, prim2, prim3)"; }
 
 int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     if (!opt || !cs || !out) return WFV_ERR_ARG;
