@@ -32,7 +32,9 @@ extern "C" {
 /* error bits accumulated on the device (wfv_get_flags) */
 #define WFV_FLAG_NAN_FLUX 1u   /* a Riemann flux was NaN (reference prints: Equations.hpp:344-359) */
 #define WFV_FLAG_BAD_THETA 2u  /* limiter theta outside [0,1] (reference prints: WENOFV.cpp:1927)  */
-#define WFV_FLAG_NAN_DT 4u     /* a NaN CFL time step                                              */
+#define WFV_FLAG_NAN_DT 4u     /* a NaN CFL time step: ends the wfv_advance / wfv_compute_dt call
+                                  that meets it (WFV_ERR_NONFINITE_DT) and is cleared when the
+                                  next such call starts, so a context given a valid state runs */
 
 /* Solver options: the config/settings.cfg keys (reference config/settings.cfg:11-24, read in
    src/WENOFV.cpp:43-63) plus the options of this implementation (defaults reproduce the

@@ -1,0 +1,100 @@
+"""Error behaviour of the C ABI (include/wenofv.h): invalid options are refused at creation
+with WFV_ERR_CONFIG and a message naming the option (the reference's std::stoi throw /
+MPI_Abort, src/WENOFV.cpp:43-63); null pointers and bad indices return WFV_ERR_ARG; a state
+that yields a NaN time step stops the run with WFV_ERR_NONFINITE_DT and the reference's
+message (src/WENOFV.cpp:502-531), after which the context still works.  Invalid inputs are
+all caught on the host; the NaN state is ordinary data to the kernels."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import wenofv
+
+pytestmark = pytest.mark.gpu
+
+PRM = dict(rho10=1.0, rho20=0.1, sound1=10.0, sound2=10.0, preb=0.0)
+
+
+def state(nx, ny):
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) / nx, (np.arange(ny) + 0.5) / ny)
+    z1 = 0.5 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    p = np.ones_like(X)
+    a, b = z1 * (1.0 + p / 100), (1 - z1) * (0.1 + p / 100)
+    return np.stack([a, b, (a + b) * 0.3, (a + b) * -0.2])
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(nx=2), "at least 3 cells"),
+    (dict(nx=0), "must be positive"),
+    (dict(scheme=3), "SCHEME"),
+    (dict(recon=5), "RECONSTRUCTIONTYPE"),
+    (dict(flux=2), "RIEMANNFLUX"),
+    (dict(rk=3), "RKMETH"),
+    (dict(precision=2), "precision"),
+    (dict(bc=4), "boundary"),
+    (dict(cfl=0.0), "CFL"),
+    (dict(nx=70, geometry_ranks=4), "divisible"),
+    (dict(domain=(1.0, 1.0, 0.0, 1.0)), "empty domain"),
+])
+def test_invalid_options_are_refused_at_creation(bad, msg):
+    kw = dict(nx=16, ny=16, domain=(0.0, 1.0, 0.0, 1.0), geometry_ranks=1, **PRM)
+    kw.update(bad)
+    nx, ny = kw.pop("nx"), kw.pop("ny")
+    with pytest.raises(wenofv.WenofvError) as e:
+        wenofv.Solver(nx, ny, **kw)
+    assert f"({wenofv.ERR_CONFIG})" in str(e.value) and msg in str(e.value), str(e.value)
+
+
+def test_null_pointers_and_bad_arguments():
+    L = wenofv.lib()
+    sol = wenofv.Solver(16, 16, geometry_ranks=1, **PRM)
+    h = sol._h
+    assert L.wfv_set_state(h, None) == wenofv.ERR_ARG
+    assert L.wfv_get_state(h, None) == wenofv.ERR_ARG
+    assert L.wfv_compute_dt(h, None) == wenofv.ERR_ARG
+    assert L.wfv_autotune(h, 0, None) == wenofv.ERR_ARG
+    assert L.wfv_set_rows_per_strip(h, -1) == wenofv.ERR_ARG
+    out = ctypes.c_void_p()
+    assert L.wfv_create(None, ctypes.byref(sol.case), ctypes.byref(out)) == wenofv.ERR_ARG
+    assert L.wfv_create(ctypes.byref(sol.opt), None, ctypes.byref(out)) == wenofv.ERR_ARG
+    # the context is untouched by the refused calls
+    sol.set_state(state(16, 16))
+    n, t, _ = sol.advance(fixed_steps=2)
+    assert n == 2 and t > 0 and sol.flags() == 0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nan_state_stops_the_run_and_the_context_recovers(precision):
+    sol = wenofv.Solver(32, 24, bc=1, geometry_ranks=1, scheme=1, flux=1, bp=1, precision=precision, **PRM)
+    U = state(32, 24)
+    bad = U.copy()
+    bad[0, 5, 7] = np.nan
+    print(f"nan state precision{precision}", flush=True)
+    sol.set_state(bad)
+    with pytest.raises(wenofv.WenofvError) as e:
+        sol.advance(fixed_steps=3)
+    assert f"error {wenofv.ERR_NONFINITE_DT}" in str(e.value) and "nan DeltaT" in str(e.value), str(e.value)
+    # a valid state runs again on the same context
+    sol.set_state(U)
+    n, t, _ = sol.advance(fixed_steps=3)
+    assert n == 3 and t > 0 and np.isfinite(sol.get_state()).all()
+
+
+def test_batch_arguments():
+    inst = [dict(nx=16, ny=16, bc=0, domain=(0.0, 1.0, 0.0, 1.0), **PRM)]
+    with pytest.raises(wenofv.WenofvError):
+        wenofv.Batch([])
+    with pytest.raises(wenofv.WenofvError):
+        wenofv.Batch([dict(inst[0], nx=2)])
+    b = wenofv.Batch(inst)
+    BL = wenofv._batch_lib()
+    U = np.ascontiguousarray(state(16, 16))
+    assert BL.wfv_batch_set_state(b._h, 1, U.ctypes.data) == wenofv.ERR_ARG
+    assert BL.wfv_batch_set_state(b._h, -1, U.ctypes.data) == wenofv.ERR_ARG
+    assert BL.wfv_batch_get_state(b._h, 0, None) == wenofv.ERR_ARG
+    with pytest.raises(wenofv.WenofvError):
+        b.set_ghost_cells(0, np.zeros((4, 24, 24)))
+    b.set_state(0, state(16, 16))
+    n, t, f = b.advance(fixed_steps=2)
+    assert (n == 2).all() and (f == 0).all()

@@ -247,6 +247,7 @@ int wfv_compute_dt(wfv_ctx *c, double *dt) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     StepCtl h{};
     HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+    h.flags &= ~WFV_FLAG_NAN_DT;  // this call's own NaN time step only
     h.dtmin_bits = ~0ull;
     HIPCHK(c, hipMemcpy(c->d_ctl, &h, sizeof h, hipMemcpyHostToDevice));
     DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags, &c->d_ctl->dtmin_bits};
@@ -313,6 +314,9 @@ int advance_impl(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int 
     HIPCHK(c, hipStreamSynchronize(c->stream));
     StepCtl h{};
     HIPCHK(c, hipMemcpy(&h, c->d_ctl, sizeof h, hipMemcpyDeviceToHost));
+    // a NaN time step ends the call that met it (and was reported by it); the other flags
+    // stay until wfv_get_flags reads them
+    h.flags &= ~WFV_FLAG_NAN_DT;
     h.dtmin_bits = ~0ull;
     h.now = t_start;
     h.t_end = t_end;
@@ -724,6 +728,7 @@ int wfv_batch_advance(wfv_batch *b, double t_end, int fixed_steps, int *nsteps, 
     std::vector<StepCtl> ctl(b->n);
     BHIPCHK(b, hipMemcpy(ctl.data(), b->d_ctl, b->n * sizeof(StepCtl), hipMemcpyDeviceToHost));
     for (auto &c : ctl) {
+        c.flags &= ~WFV_FLAG_NAN_DT;  // as in wfv_advance
         c.dtmin_bits = ~0ull;
         c.now = 0.0;
         c.t_end = t_end;
