@@ -2,8 +2,9 @@
 with WFV_ERR_CONFIG and a message naming the option (the reference's std::stoi throw /
 MPI_Abort, src/WENOFV.cpp:43-63); null pointers and bad indices return WFV_ERR_ARG; a state
 that yields a NaN time step stops the run with WFV_ERR_NONFINITE_DT and the reference's
-message (src/WENOFV.cpp:502-531), after which the context still works.  Invalid inputs are
-all caught on the host; the NaN state is ordinary data to the kernels."""
+message (src/WENOFV.cpp:502-531), after which the context (or batch) still works; two
+contexts on one GPU do not interact.  Invalid inputs are all caught on the host; the NaN
+state is ordinary data to the kernels."""
 import ctypes
 
 import numpy as np
@@ -98,3 +99,45 @@ def test_batch_arguments():
     b.set_state(0, state(16, 16))
     n, t, f = b.advance(fixed_steps=2)
     assert (n == 2).all() and (f == 0).all()
+
+
+def test_batch_recovers_after_a_nan_instance():
+    inst = [dict(nx=16, ny=16, bc=0, domain=(0.0, 1.0, 0.0, 1.0), **PRM),
+            dict(nx=24, ny=20, bc=1, domain=(0.0, 1.0, 0.0, 1.0), **PRM)]
+    b = wenofv.Batch(inst, scheme=1, flux=1, bp=1)
+    good = [state(16, 16), state(24, 20)]
+    bad = good[1].copy()
+    bad[2, 3, 4] = np.nan
+    b.set_state(0, good[0])
+    b.set_state(1, bad)
+    print("batch nan instance", flush=True)
+    with pytest.raises(wenofv.WenofvError):
+        b.advance(fixed_steps=2)
+    b.set_state(0, good[0])
+    b.set_state(1, good[1])
+    n, t, f = b.advance(fixed_steps=2)
+    assert (n == 2).all() and np.isfinite(b.get_state(1)).all()
+
+
+def test_contexts_are_independent():
+    """Two solvers on one GPU, advanced in alternation, each equal to a solver run alone
+    (one stream per context, nothing shared)."""
+    opts = dict(geometry_ranks=1, scheme=1, flux=1, bp=1, **PRM)
+    a = wenofv.Solver(32, 24, bc=1, **opts)
+    b = wenofv.Solver(20, 28, bc=0, precision=1, **opts)
+    Ua, Ub = state(32, 24), state(20, 28)[:, ::-1].copy()
+    a.set_state(Ua)
+    b.set_state(Ub)
+    for _ in range(3):
+        a.advance(fixed_steps=2)
+        b.advance(fixed_steps=3)
+    solo_a = wenofv.Solver(32, 24, bc=1, **opts)
+    solo_a.set_state(Ua)
+    for _ in range(3):
+        solo_a.advance(fixed_steps=2)
+    solo_b = wenofv.Solver(20, 28, bc=0, precision=1, **opts)
+    solo_b.set_state(Ub)
+    for _ in range(3):
+        solo_b.advance(fixed_steps=3)
+    np.testing.assert_array_equal(a.get_state(), solo_a.get_state())
+    np.testing.assert_array_equal(b.get_state(), solo_b.get_state())
