@@ -64,7 +64,11 @@ int main() {
     const int nxs[] = {1, 5, 6, 7, 16, 20, 25, 26, 27, 31, 32, 50, 56, 57, 58, 59, 63, 64, 70, 100,
                        112, 115, 116, 117, 128, 170, 174, 200, 256, 300, 512, 1000, 1024};
     const int nys[] = {1, 2, 3, 7, 16, 33, 64, 100, 128, 257};
-    const int tys[] = {1, 2, 3, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+    // every rows-per-strip value up to 40 (the autotuner's occupancy candidates can be any
+    // value) and the larger ones it tries or may pick
+    std::vector<int> tys;
+    for (int t = 1; t <= 40; ++t) tys.push_back(t);
+    for (int t : {44, 48, 50, 56, 60, 64, 74, 76, 80, 96, 100, 112, 127, 128}) tys.push_back(t);
     int cases = 0;
     for (int halo : {3, 4})
         for (int nx : nxs)

@@ -1,8 +1,9 @@
 """The stage kernels' work decomposition, checked on the host: tests/strip_plan_check.cpp
 compiles the kernels' own strip_plan / segment_of (hipcc, host code only) and verifies, over
-8,580 grid shapes, rows-per-strip choices and both stencil halos, that every cell is owned
-exactly once and that every lane a cell's reconstruction and face fluxes read stays in its
-segment, on the adjacent column, with the same rows."""
+35,640 combinations of grid shape, rows per strip (every value up to 40 and the larger ones
+the autotuner tries) and stencil halo, that every cell is owned exactly once and that every
+lane a cell's reconstruction and face fluxes read stays in its segment, on the adjacent
+column, with the same rows."""
 import os
 import shutil
 import subprocess
