@@ -417,12 +417,18 @@ def run_single(a, torch, wenofv, cases):
         rng = np.random.default_rng(13)
         *_, Up, _, _ = run(0, wl.U0 * (1 + 1e-13 * rng.standard_normal(wl.U0.shape)))
         spread, _, _ = field_errors(Up, U64, wl.params)
+        # and by 6e-8 (FP32's unit roundoff): the FP64 solution's response to one float
+        # rounding of its input, the floor for any arithmetic that rounds to FP32
+        *_, Up8, _, _ = run(0, wl.U0 * (1 + 6e-8 * rng.standard_normal(wl.U0.shape)))
+        spread8, _, _ = field_errors(Up8, U64, wl.params)
         worst = max(errs[k] for k in gate)
         spread_max = max(spread[k] for k in gate)
+        spread8_max = max(spread8[k] for k in gate)
         out["accuracy"] = {"rel_l1_per_field": errs, "rel_linf": linf, "max_rel_l1": worst,
                            "gate_fields": list(gate), "tolerance": 1e-5, "within_tolerance": worst <= 1e-5,
-                           "fp64_self_spread_1e-13": spread_max,
-                           "within_tolerance_or_10x_fp64_spread": worst <= max(1e-5, 10 * spread_max)}
+                           "fp64_self_spread_1e-13": spread_max, "fp64_self_spread_6e-8": spread8_max,
+                           "within_tolerance_or_10x_fp64_spread": worst <= max(1e-5, 10 * spread_max),
+                           "within_tolerance_or_fp64_6e-8_spread": worst <= max(1e-5, spread8_max)}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline_oracle(wl.nx, wl.ny, wl.domain, wl.bc, wl.params, wl.opts, wl.U0,
                                                   a.cpu_seconds, wl.name)
