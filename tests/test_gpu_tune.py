@@ -40,6 +40,24 @@ def test_solver_autotune_keeps_results(precision, tmp_path, monkeypatch):
     assert again.rows_per_strip() == res["rows_per_strip"]
 
 
+def test_solver_autotune_on_a_medium_grid_keeps_results(tmp_path, monkeypatch):
+    """1024 x 512: the tuner adds its occupancy candidates (strip heights whose wavefront
+    count fills whole rounds of the device, here 3, 5 and 10 rows) to the fixed list."""
+    monkeypatch.setenv("WENOFV_TUNING_CACHE", str(tmp_path / "tune.json"))
+    wl = cases.liquid_column(nx=1024, ny=512, precision=0)
+    print("solver autotune 1024x512", flush=True)
+    ref = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs())
+    ref.set_state(wl.U0)
+    ref.advance(fixed_steps=3)
+    sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs())
+    sol.set_state(wl.U0)
+    res = sol.autotune(trials=1, cache=False)
+    print(res, flush=True)
+    np.testing.assert_array_equal(sol.get_state(), wl.U0)
+    sol.advance(fixed_steps=3)
+    np.testing.assert_array_equal(sol.get_state(), ref.get_state())
+
+
 @pytest.mark.parametrize("rows", [1, 2, 5, 8, 33, 200])
 def test_solver_rows_per_strip_is_invisible(rows):
     wl = cases.batch_instance(dict(cases.batch_specs(1, seed=6)[0], nx=90, ny=77), precision=0)
