@@ -9,8 +9,9 @@
 #   oracle/_ref/refmain_gpu    : the reference's unmodified main program (src/main.cpp)
 #                                built against this build's CWENOFV facade and libwenofv.so
 # Flags follow the reference makefile (-O3, no -march: no FMA).  Outputs go only to
-# oracle/_ref/ (git-ignored).  Development container only: the reference is not present
-# on the GPU machines, where only the committed fixtures are used.
+# oracle/_ref/ (git-ignored, and excluded from the tree uploaded to GPU machines by
+# .gpurunignore).  Development container only: neither the reference nor anything built
+# from it goes to the GPU machines, where only the committed fixtures are used.
 set -euo pipefail
 REF=${WENOFV_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)

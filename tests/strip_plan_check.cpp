@@ -60,7 +60,30 @@ static int check(int nx, int ny, int ty, int halo) {
     return 0;
 }
 
+// The autotuner's occupancy candidates (occupancy_candidates) on a 1024 x 512 grid of the
+// 5-point schemes (halo 3) and a 256-CU device: 18 strips across; 10 rows fill one round of
+// one wavefront per SIMD (936 <= 1024 wavefronts; 9 rows would need 1,026), 5 rows two such
+// rounds or one of two per SIMD (1,854 <= 2,048), 4 and 3 rows three and four rounds
+// (2,304 <= 3,072, 3,078 <= 4,096), 2 rows three rounds of two per SIMD (4,608 <= 6,144).
+static int check_occupancy() {
+    int out[8];
+    const int n = wfv::occupancy_candidates(1024, 512, 3, 256, out);
+    const int want[8] = {10, 5, 4, 3, 5, 3, 2, 2};
+    if (n != 8) return fail("occupancy candidates: count", 1024, 512, n, 3, 0, 0);
+    for (int k = 0; k < 8; ++k)
+        if (out[k] != want[k]) return fail("occupancy candidates: value", 1024, 512, out[k], 3, k, want[k]);
+    // every candidate's wavefronts fit its rounds, and one row fewer would not
+    const int caps[8] = {1024, 2048, 3072, 4096, 2048, 4096, 6144, 8192};
+    for (int k = 0; k < 8; ++k) {
+        if (wfv::strip_plan(1024, 512, out[k], 3).total > caps[k]) return fail("occupancy: over", 1024, 512, out[k], 3, k, 0);
+        if (out[k] > 1 && wfv::strip_plan(1024, 512, out[k] - 1, 3).total <= caps[k])
+            return fail("occupancy: not the fewest rows", 1024, 512, out[k], 3, k, 0);
+    }
+    return 0;
+}
+
 int main() {
+    if (check_occupancy()) return 1;
     const int nxs[] = {1, 5, 6, 7, 16, 20, 25, 26, 27, 31, 32, 50, 56, 57, 58, 59, 63, 64, 70, 100,
                        112, 115, 116, 117, 128, 170, 174, 200, 256, 300, 512, 1000, 1024};
     const int nys[] = {1, 2, 3, 7, 16, 33, 64, 100, 128, 257};

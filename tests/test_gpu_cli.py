@@ -1,7 +1,9 @@
 """The command-line drivers on the GPU: this build's `wenofv_run` and the reference's own,
 unmodified main program built against the CWENOFV facade (oracle/_ref/refmain_gpu, from
 oracle/build_ref.sh).  Both read config/settings.cfg and their accuracy output reproduces
-the norms the reference itself wrote for the SMOOTH case."""
+the norms the reference itself wrote for the SMOOTH case.  The reference's main program is
+built from the reference sources in the development container only and is never uploaded
+with the tree, so on a machine without it that test reports as skipped."""
 import glob
 import os
 import subprocess

@@ -141,3 +141,68 @@ def test_contexts_are_independent():
         solo_b.advance(fixed_steps=3)
     np.testing.assert_array_equal(a.get_state(), solo_a.get_state())
     np.testing.assert_array_equal(b.get_state(), solo_b.get_state())
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_rk1_run_of_no_steps_leaves_the_state_unchanged(precision):
+    """SSP-RK1 writes its stage into a second register and copies it back; a run that takes
+    no step (t_end = t_start, or a state whose first time step is NaN) must leave the state
+    as it was, as the reference's loop does when it takes no step (src/WENOFV.cpp:318-332)."""
+    nx, ny = 40, 24
+    U0 = state(nx, ny)
+    sol = wenofv.Solver(nx, ny, rk=0, precision=precision, bc=1, geometry_ranks=1, scheme=1, flux=1, bp=1, **PRM)
+    print(f"rk1 zero steps precision{precision}", flush=True)
+    sol.set_state(U0)
+    n, t, _ = sol.advance(t_end=0.0, t_start=0.0)
+    assert n == 0 and t == 0.0
+    np.testing.assert_array_equal(sol.get_state(), U0)
+    # a NaN first time step: the run stops before its first step, the state stays
+    bad = U0.copy()
+    bad[1, 3, 5] = np.nan
+    sol.set_state(bad)
+    with pytest.raises(wenofv.WenofvError):
+        sol.advance(fixed_steps=2)
+    assert np.array_equal(sol.get_state(), bad, equal_nan=True)
+    # and a real step still changes it
+    sol.set_state(U0)
+    n, t, _ = sol.advance(fixed_steps=1)
+    assert n == 1 and not np.array_equal(sol.get_state(), U0)
+
+
+def test_rk1_batch_run_of_no_steps_leaves_the_states_unchanged():
+    inst = [dict(nx=16, ny=16, bc=0, domain=(0.0, 1.0, 0.0, 1.0), **PRM),
+            dict(nx=24, ny=20, bc=1, domain=(0.0, 1.0, 0.0, 1.0), **PRM)]
+    b = wenofv.Batch(inst, rk=0, scheme=1, flux=1, bp=1)
+    good = [state(16, 16), state(24, 20)]
+    for k in range(2):
+        b.set_state(k, good[k])
+    print("rk1 batch zero steps", flush=True)
+    n, t, f = b.advance(t_end=0.0)
+    assert (n == 0).all()
+    for k in range(2):
+        np.testing.assert_array_equal(b.get_state(k), good[k])
+    # one instance stops at a NaN time step before its first step; the other runs
+    bad = good[1].copy()
+    bad[0, 2, 2] = np.nan
+    b.set_state(1, bad)
+    with pytest.raises(wenofv.WenofvError):
+        b.advance(fixed_steps=2)
+    assert np.array_equal(b.get_state(1), bad, equal_nan=True)
+    assert not np.array_equal(b.get_state(0), good[0])
+
+
+def test_rows_per_strip_beyond_the_grid_is_the_whole_grid():
+    L = wenofv.lib()
+    sol = wenofv.Solver(40, 24, geometry_ranks=1, **PRM)
+    assert L.wfv_set_rows_per_strip(sol._h, 2 ** 31 - 1) == wenofv.OK
+    assert sol.rows_per_strip() == 24
+    sol.set_state(state(40, 24))
+    n, t, _ = sol.advance(fixed_steps=2)
+    assert n == 2 and sol.flags() == 0
+    assert L.wfv_set_rows_per_strip(sol._h, 0) == wenofv.OK  # 0: the default, as for a batch
+    assert 1 <= sol.rows_per_strip() <= 24
+    inst = [dict(nx=16, ny=16, bc=0, domain=(0.0, 1.0, 0.0, 1.0), **PRM)]
+    b = wenofv.Batch(inst, rows_per_strip=1000)
+    assert b.rows_per_strip(0) == 16
+    b.set_rows_per_strip(0, 2 ** 31 - 1)
+    assert b.rows_per_strip(0) == 16

@@ -42,7 +42,8 @@ def test_solver_autotune_keeps_results(precision, tmp_path, monkeypatch):
 
 def test_solver_autotune_on_a_medium_grid_keeps_results(tmp_path, monkeypatch):
     """1024 x 512: the tuner adds its occupancy candidates (strip heights whose wavefront
-    count fills whole rounds of the device, here 3, 5 and 10 rows) to the fixed list."""
+    count fills whole rounds of the device; on 256 CUs 2, 3, 4, 5 and 10 rows, which
+    tests/strip_plan_check.cpp checks on the host) to the fixed list."""
     monkeypatch.setenv("WENOFV_TUNING_CACHE", str(tmp_path / "tune.json"))
     wl = cases.liquid_column(nx=1024, ny=512, precision=0)
     print("solver autotune 1024x512", flush=True)
@@ -114,3 +115,22 @@ def test_batch_schedule_is_invisible(persistent):
         x.advance(fixed_steps=3)
     for k in range(len(wls)):
         np.testing.assert_array_equal(a.get_state(k), b.get_state(k))
+
+
+def test_autotune_leaves_recorded_limiter_flags_alone():
+    """The tuner's timed stages write neither the limiter flags a context records nor its
+    error bits: the flags read after tuning are those of the last stage of the run."""
+    wl = cases.liquid_column(nx=160, ny=96, precision=0)
+    sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs())
+    sol.set_state(wl.U0)
+    L = wenofv.lib()
+    assert L.wfv_record_limiter_flags(sol._h, 1) == wenofv.OK
+    print("autotune keeps limiter flags", flush=True)
+    sol.advance(fixed_steps=4)
+    before = np.empty((wl.ny, wl.nx))
+    assert L.wfv_get_limiter_flags(sol._h, before.ctypes.data) == wenofv.OK
+    assert before.sum() > 0  # the shock front limits some cells
+    sol.autotune(trials=1, cache=False)
+    after = np.empty((wl.ny, wl.nx))
+    assert L.wfv_get_limiter_flags(sol._h, after.ctypes.data) == wenofv.OK
+    np.testing.assert_array_equal(after, before)

@@ -22,10 +22,104 @@ using namespace wfv;
 
 
 namespace {
-thread_local std::string g_create_error;  // reported by wfv_last_error(NULL)
+// failure of the last call without a context on this thread (wfv_last_error(NULL)); a fixed
+// buffer, so that recording it allocates nothing
+thread_local char g_create_error[256];
 }  // namespace
 
 namespace wfv {
+
+void set_thread_error(const char *msg) {
+    std::snprintf(g_create_error, sizeof g_create_error, "%s", msg ? msg : "");
+}
+
+const char *check_options(const wfv_options *opt, const wfv_case *cs) {
+    if (opt->elem_num_x < 1 || opt->elem_num_y < 1) return "ELEMNUMX/ELEMNUMY must be positive";
+    if (opt->elem_num_x < 3 || opt->elem_num_y < 3) return "grids need at least 3 cells per direction";
+    if (opt->rk_method < 0 || opt->rk_method > 2) return "RKMETH must be 0, 1 or 2";
+    if (opt->scheme < 0 || opt->scheme > 2) return "SCHEME must be 0 (WENO-JS), 1 (WENO-Z) or 2 (WENO-EXP)";
+    if (opt->riemann_flux < 0 || opt->riemann_flux > 1) return "RIEMANNFLUX must be 0 or 1";
+    if (opt->reconstruction < 0 || opt->reconstruction > 4) return "RECONSTRUCTIONTYPE must be 0..4";
+    if (opt->precision < 0 || opt->precision > 1) return "precision must be 0 (fp64) or 1 (mixed)";
+    if (opt->boundary < 0 || opt->boundary > 3)
+        return "boundary must be 0 (periodic), 1 (slip walls), 2 (no-slip walls) or 3 (fixed ghost cells)";
+    if (!(opt->cfl > 0)) return "CFL must be positive";
+    if (opt->geometry_ranks < 0 || (opt->geometry_ranks > 1 && opt->elem_num_x % opt->geometry_ranks))
+        return "ELEMNUMX must be divisible by the number of ranks of the reference grid";
+    if (!(cs->xR > cs->xL) || !(cs->yR > cs->yL)) return "empty domain";
+    return nullptr;
+}
+
+Phys make_phys(const wfv_case &cs) {
+    Phys P{};
+    P.rho10 = cs.rho10;
+    P.rho20 = cs.rho20;
+    P.c1_2 = cs.sound1 * cs.sound1;
+    P.c2_2 = cs.sound2 * cs.sound2;
+    P.preb = cs.preb;
+    P.q = cs.rho20 * P.c2_2 - cs.rho10 * P.c1_2;
+    P.ic1_2 = 1.0 / P.c1_2;
+    P.ic2_2 = 1.0 / P.c2_2;
+    return P;
+}
+
+StageArgs stage_base(const wfv_options &opt, const wfv_case &cs, int nx, int ny, int ty, const double *dxC,
+                     const double *dxW, const double *dyC, const double *ghost, StepCtl *ctl) {
+    StageArgs A{};
+    A.dxC = dxC;
+    A.dxW = dxW;
+    A.dyC = dyC;
+    A.P = make_phys(cs);
+    A.dX = (cs.xR - cs.xL) / nx;
+    A.dY = (cs.yR - cs.yL) / ny;
+    A.cfl = opt.cfl;
+    A.gravity = cs.gravity;
+    A.nx = nx;
+    A.ny = ny;
+    A.bc = opt.boundary;
+    A.ghost = ghost;
+    A.theta_flag = nullptr;
+    A.flux = opt.riemann_flux;
+    A.bp = opt.pp_limiter;
+    A.ty = ty;
+    A.sp = strip_plan(nx, ny, ty, strip_halo(opt.scheme, opt.reconstruction));
+    A.dx4 = std::pow(A.dX, 4);  // WENO-EXP's threshold, as the reference's pow(m_deltaX, 4)
+    A.flags = &ctl->flags;
+    return A;
+}
+
+hipError_t enqueue_rk_step(const StageArgs &base, double *const reg[3], const wfv_options &opt, double dt,
+                           const double *dt_dev, const int *active, unsigned long long *dtmin, hipStream_t s) {
+    StagePlan pl[3];
+    const int ns = rk_plan(opt.rk_method, pl);
+    for (int st = 0; st < ns; ++st) {
+        StageArgs A = base;
+        A.U = reg[pl[st].in];
+        A.Un = pl[st].base < 0 ? nullptr : reg[pl[st].base];
+        A.Uo = reg[pl[st].out];
+        A.a = pl[st].a;
+        A.b = pl[st].b;
+        A.c = pl[st].c;
+        A.dt = dt;
+        A.dt_ptr = dt_dev;
+        A.active = active;
+        A.dtmin = (st == ns - 1 && opt.rk_method != 0) ? dtmin : nullptr;
+        hipError_t e = launch_stage(A, opt.precision, opt.reconstruction, opt.scheme, s);
+        if (e != hipSuccess) return e;
+    }
+    if (opt.rk_method == 0) {
+        // RK1 writes register 1; copy back so that register 0 stays the state (skipped, like
+        // the stage, once the device loop has finished)
+        hipError_t e = launch_copy_state(reg[0], reg[1], 4 * (size_t)base.nx * base.ny, active, s);
+        if (e != hipSuccess) return e;
+        if (dtmin) {
+            DtArgs D{reg[0], base.P, base.dX, base.dY, base.cfl, base.nx, base.ny, base.flags, dtmin};
+            e = launch_dt(D, s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
 
 int fail(wfv_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
@@ -38,26 +132,8 @@ int bfail(wfv_batch *b, int code, const std::string &msg) {
 }
 
 StageArgs base_args(wfv_ctx *c) {
-    StageArgs A{};
-    A.dxC = c->d_dxC;
-    A.dxW = c->d_dxW;
-    A.dyC = c->d_dyC;
-    A.P = c->P;
-    A.dX = c->dX;
-    A.dY = c->dY;
-    A.cfl = c->opt.cfl;
-    A.gravity = c->cs.gravity;
-    A.nx = c->nx;
-    A.ny = c->ny;
-    A.bc = c->opt.boundary;
-    A.ghost = c->d_ghost;
+    StageArgs A = stage_base(c->opt, c->cs, c->nx, c->ny, c->ty, c->d_dxC, c->d_dxW, c->d_dyC, c->d_ghost, c->d_ctl);
     A.theta_flag = c->d_theta_flag;
-    A.flux = c->opt.riemann_flux;
-    A.bp = c->opt.pp_limiter;
-    A.ty = c->ty;
-    A.sp = strip_plan(c->nx, c->ny, c->ty, strip_halo(c->opt.scheme, c->opt.reconstruction));
-    A.dx4 = std::pow(c->dX, 4);  // WENO-EXP's threshold, as the reference's pow(m_deltaX, 4)
-    A.flags = &c->d_ctl->flags;
     return A;
 }
 
@@ -84,33 +160,8 @@ namespace {
 
 // enqueue one RK step; dt from the host (dt_dev == nullptr) or from the device loop
 int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active, bool want_dtmin) {
-    StagePlan pl[3];
-    const int ns = rk_plan(c->opt.rk_method, pl);
-    for (int s = 0; s < ns; ++s) {
-        StageArgs A = base_args(c);
-        A.U = c->reg[pl[s].in];
-        A.Un = pl[s].base < 0 ? nullptr : c->reg[pl[s].base];
-        A.Uo = c->reg[pl[s].out];
-        A.a = pl[s].a;
-        A.b = pl[s].b;
-        A.c = pl[s].c;
-        A.dt = dt;
-        A.dt_ptr = dt_dev;
-        A.active = active;
-        const bool last = s == ns - 1;
-        A.dtmin = (last && want_dtmin && c->opt.rk_method != 0) ? &c->d_ctl->dtmin_bits : nullptr;
-        HIPCHK(c, launch_stage(A, c->opt.precision, c->opt.reconstruction, c->opt.scheme, c->stream));
-    }
-    if (c->opt.rk_method == 0) {
-        // RK1 writes register 1; copy back so that register 0 stays the state
-        HIPCHK(c, hipMemcpyAsync(c->reg[0], c->reg[1], 4 * c->nc * sizeof(double), hipMemcpyDeviceToDevice,
-                                 c->stream));
-        if (want_dtmin) {
-            DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags,
-                     &c->d_ctl->dtmin_bits};
-            HIPCHK(c, launch_dt(D, c->stream));
-        }
-    }
+    HIPCHK(c, enqueue_rk_step(base_args(c), c->reg, c->opt, dt, dt_dev, active,
+                              want_dtmin ? &c->d_ctl->dtmin_bits : nullptr, c->stream));
     return WFV_OK;
 }
 
@@ -126,37 +177,16 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     wfv_ctx *c = new wfv_ctx();
     c->opt = *opt;
     c->cs = *cs;
-    std::string why;
-    if (opt->elem_num_x < 1 || opt->elem_num_y < 1) why = "ELEMNUMX/ELEMNUMY must be positive";
-    else if (opt->elem_num_x < 3 || opt->elem_num_y < 3) why = "grids need at least 3 cells per direction";
-    else if (opt->rk_method < 0 || opt->rk_method > 2) why = "RKMETH must be 0, 1 or 2";
-    else if (opt->scheme < 0 || opt->scheme > 2) why = "SCHEME must be 0 (WENO-JS), 1 (WENO-Z) or 2 (WENO-EXP)";
-    else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
-    else if (opt->reconstruction < 0 || opt->reconstruction > 4) why = "RECONSTRUCTIONTYPE must be 0..4";
-    else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 (fp64) or 1 (mixed)";
-    else if (opt->boundary < 0 || opt->boundary > 3)
-        why = "boundary must be 0 (periodic), 1 (slip walls), 2 (no-slip walls) or 3 (fixed ghost cells)";
-    else if (!(opt->cfl > 0)) why = "CFL must be positive";
-    else if (opt->geometry_ranks < 0 || (opt->geometry_ranks > 1 && opt->elem_num_x % opt->geometry_ranks))
-        why = "ELEMNUMX must be divisible by the number of ranks of the reference grid";
-    else if (!(cs->xR > cs->xL) || !(cs->yR > cs->yL)) why = "empty domain";
-    if (!why.empty()) {
+    if (const char *why = check_options(opt, cs)) {
         delete c;
-        g_create_error = why;
+        set_thread_error(why);
         return WFV_ERR_CONFIG;
     }
     c->nx = opt->elem_num_x;
     c->ny = opt->elem_num_y;
     c->nc = (size_t)c->nx * c->ny;
-    c->ty = opt->rows_per_strip > 0 ? opt->rows_per_strip : default_rows_per_strip(c->nx, c->ny);
-    c->P.rho10 = cs->rho10;
-    c->P.rho20 = cs->rho20;
-    c->P.c1_2 = cs->sound1 * cs->sound1;
-    c->P.c2_2 = cs->sound2 * cs->sound2;
-    c->P.preb = cs->preb;
-    c->P.q = cs->rho20 * c->P.c2_2 - cs->rho10 * c->P.c1_2;
-    c->P.ic1_2 = 1.0 / c->P.c1_2;
-    c->P.ic2_2 = 1.0 / c->P.c2_2;
+    c->ty = opt->rows_per_strip > 0 ? std::min(opt->rows_per_strip, c->ny) : default_rows_per_strip(c->nx, c->ny);
+    c->P = make_phys(*cs);
     c->dX = (cs->xR - cs->xL) / c->nx;
     c->dY = (cs->yR - cs->yL) / c->ny;
     c->dxC.resize(c->nx);
@@ -166,7 +196,7 @@ int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
                 c->dxC.data(), c->dxW.data(), c->dyC.data(), nullptr, nullptr);
     auto bad = [&](hipError_t e) {
         if (e == hipSuccess) return false;
-        g_create_error = hipGetErrorString(e);
+        set_thread_error(hipGetErrorString(e));
         wfv_destroy(c);
         return true;
     };
@@ -215,7 +245,7 @@ void wfv_destroy(wfv_ctx *c) {
     delete c;
 }
 
-const char *wfv_last_error(const wfv_ctx *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+const char *wfv_last_error(const wfv_ctx *c) { return c ? c->err.c_str() : g_create_error; }
 
 int wfv_set_state(wfv_ctx *c, const double *U) {
     if (!c || !U) return WFV_ERR_ARG;
@@ -531,10 +561,8 @@ int batch_enqueue_step(wfv_batch *b) {
                                       b->opt.riemann_flux, b->grid, b->stream));
     }
     if (b->opt.rk_method == 0) {
-        for (int i = 0; i < b->n; ++i)
-            BHIPCHK(b, hipMemcpyAsync(b->hinst[i].reg[0], b->hinst[i].reg[1],
-                                      4 * (size_t)b->inst[i].nx * b->inst[i].ny * sizeof(double),
-                                      hipMemcpyDeviceToDevice, b->stream));
+        // register 1 back into the state of every instance whose loop is still running
+        BHIPCHK(b, launch_batch_copy_state(b->d_inst, b->n, b->max_cells, b->stream));
         BHIPCHK(b, launch_batch_dt(b->d_inst, b->n, b->max_cells, b->stream));
     }
     return WFV_OK;
@@ -562,7 +590,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         else if (opt->geometry_ranks > 1 && I.nx % opt->geometry_ranks) why = "nx not divisible by geometry_ranks";
     }
     if (!why.empty()) {
-        g_create_error = why;
+        set_thread_error(why.c_str());
         return WFV_ERR_CONFIG;
     }
     wfv_batch *b = new wfv_batch();
@@ -590,12 +618,12 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         b->total_cells += (size_t)inst[i].nx * inst[i].ny;
         geo_total += 2 * (size_t)inst[i].nx + inst[i].ny;
         b->max_cells = std::max(b->max_cells, inst[i].nx * inst[i].ny);
-        b->ty[i] = ty0;
+        b->ty[i] = std::min(ty0, inst[i].ny);  // strips longer than the grid are the whole grid
     }
     b->default_ty = ty0;
     auto bad = [&](hipError_t e) {
         if (e == hipSuccess) return false;
-        g_create_error = hipGetErrorString(e);
+        set_thread_error(hipGetErrorString(e));
         wfv_batch_destroy(b);
         return true;
     };
@@ -625,14 +653,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
         I.dyC = I.dxW + in.nx;
         go += 2 * (size_t)in.nx + in.ny;
         I.ctl = b->d_ctl + i;
-        I.P.rho10 = in.cs.rho10;
-        I.P.rho20 = in.cs.rho20;
-        I.P.c1_2 = in.cs.sound1 * in.cs.sound1;
-        I.P.c2_2 = in.cs.sound2 * in.cs.sound2;
-        I.P.preb = in.cs.preb;
-        I.P.q = in.cs.rho20 * I.P.c2_2 - in.cs.rho10 * I.P.c1_2;
-        I.P.ic1_2 = 1.0 / I.P.c1_2;
-        I.P.ic2_2 = 1.0 / I.P.c2_2;
+        I.P = make_phys(in.cs);
         I.dX = (in.cs.xR - in.cs.xL) / in.nx;
         I.dY = (in.cs.yR - in.cs.yL) / in.ny;
         I.dx4 = std::pow(I.dX, 4);
@@ -651,7 +672,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     }
     if (bad(hipMemcpy(b->d_ctl, ctl.data(), n * sizeof(StepCtl), hipMemcpyHostToDevice))) return WFV_ERR_HIP;
     if (batch_layout(b) != WFV_OK) {
-        g_create_error = b->err;
+        set_thread_error(b->err.c_str());
         wfv_batch_destroy(b);
         return WFV_ERR_HIP;
     }
@@ -673,7 +694,7 @@ void wfv_batch_destroy(wfv_batch *b) {
     delete b;
 }
 
-const char *wfv_batch_last_error(const wfv_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+const char *wfv_batch_last_error(const wfv_batch *b) { return b ? b->err.c_str() : g_create_error; }
 
 int wfv_batch_set_state(wfv_batch *b, int i, const double *U) {
     if (!b || !U || i < 0 || i >= b->n) return WFV_ERR_ARG;
@@ -712,7 +733,8 @@ int wfv_batch_set_stream(wfv_batch *b, void *hip_stream) {
 int wfv_batch_set_rows_per_strip(wfv_batch *b, int i, int rows) {
     if (!b || i < 0 || i >= b->n || rows < 0) return WFV_ERR_ARG;
     BHIPCHK(b, hipStreamSynchronize(b->stream));
-    b->ty[i] = rows > 0 ? rows : b->default_ty;
+    // 0: the batch default; strips longer than the grid are the whole grid
+    b->ty[i] = std::min(rows > 0 ? rows : b->default_ty, b->inst[i].ny);
     return batch_layout(b);
 }
 

@@ -139,6 +139,40 @@ __global__ __launch_bounds__(DT_THREADS) void batch_dt_kernel(const BatchInst *i
     }
 }
 
+// SSP-RK1 writes its stage into register 1 and this copies it back into the state.  A stage
+// enqueued after the device loop has finished writes nothing, so neither does its copy.
+__global__ __launch_bounds__(DT_THREADS) void copy_state_kernel(double *dst, const double *src, size_t n,
+                                                                const int *active) {
+    if (active && *active == 0) return;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
+// the same for every instance of a batch (blockIdx.y = instance)
+__global__ __launch_bounds__(DT_THREADS) void batch_copy_state_kernel(const BatchInst *inst) {
+    const BatchInst &I = inst[blockIdx.y];
+    if (I.ctl->active == 0) return;
+    const size_t n = 4 * (size_t)I.nx * I.ny;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        I.reg[0][k] = I.reg[1][k];
+}
+
+hipError_t launch_copy_state(double *dst, const double *src, size_t n, const int *active, hipStream_t s) {
+    size_t blocks = (n + DT_THREADS - 1) / DT_THREADS;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_state_kernel, dim3((unsigned)blocks), dim3(DT_THREADS), 0, s, dst, src, n, active);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_copy_state(const BatchInst *inst, int n, int max_cells, hipStream_t s) {
+    int bx = (int)((4 * (size_t)max_cells + DT_THREADS - 1) / DT_THREADS);
+    if (bx > 256) bx = 256;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(batch_copy_state_kernel, dim3(bx, n), dim3(DT_THREADS), 0, s, inst);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_step_control(StepCtl *ctl, int n, hipStream_t s) {
     hipLaunchKernelGGL(batch_step_control_kernel, dim3((n + 63) / 64), dim3(64), 0, s, ctl, n);
     return hipGetLastError();

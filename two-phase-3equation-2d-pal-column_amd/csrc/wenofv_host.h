@@ -68,6 +68,21 @@ struct StagePlan {
 int rk_plan(int rk, StagePlan *s);
 
 int fail(wfv_ctx *c, int code, const std::string &msg);
+// the message wfv_last_error(NULL) returns on this thread (copied into a fixed buffer)
+void set_thread_error(const char *msg);
+// nullptr if the options and case are valid, else why not
+const char *check_options(const wfv_options *opt, const wfv_case *cs);
+// equation-of-state parameters of a case
+Phys make_phys(const wfv_case &cs);
+// stage arguments of a grid without registers, RK coefficients or dt: widths dxC, dxW
+// [nx] and dyC [ny] (device), the fixed ghost ring or nullptr, error bits in ctl->flags
+StageArgs stage_base(const wfv_options &opt, const wfv_case &cs, int nx, int ny, int ty, const double *dxC,
+                     const double *dxW, const double *dyC, const double *ghost, StepCtl *ctl);
+// enqueue the stages of one RK step from reg[0] over reg[1], reg[2] (SSP-RK1: reg[1] copied
+// back into reg[0]); dt from the host (dt_dev == nullptr) or the device loop (dt_dev,
+// active); dtmin (optional) receives the CFL minimum of the new state
+hipError_t enqueue_rk_step(const StageArgs &base, double *const reg[3], const wfv_options &opt, double dt,
+                           const double *dt_dev, const int *active, unsigned long long *dtmin, hipStream_t s);
 
 // a padded state [4][ny+2G][nx+2G] -> its ghost ring, SoA [4][ring_size(nx, ny)]
 inline void pack_ghost_ring(int nx, int ny, const double *Upad, double *ring) {

@@ -82,6 +82,27 @@ __host__ __device__ inline StripPlan strip_plan(int nx, int ny, int ty, int halo
     return p;
 }
 
+// Rows-per-strip candidates that fit a grid's wavefronts to whole rounds of the device.  A
+// grid of a few hundred thousand to a few million cells gives fewer wavefronts than the
+// device holds at once, or a last round that is nearly empty, and a launch lasts as long as
+// its most loaded SIMD.  For 1 and 2 resident wavefronts per SIMD (4 and 8 per CU) and 1-4
+// rounds: the fewest rows per strip whose wavefront count fits that many rounds.  Writes at
+// most 8 values (unsorted, possibly repeated) to out and returns their count.
+__host__ __device__ inline int occupancy_candidates(int nx, int ny, int halo, int cus, int out[8]) {
+    int n = 0;
+    const int top = ny < 128 ? ny : 128;
+    for (int per_cu = 4; per_cu <= 8; per_cu += 4)
+        for (int rounds = 1; rounds <= 4; ++rounds) {
+            const long long cap = (long long)rounds * per_cu * cus;
+            for (int ty = 1; ty <= top; ++ty)
+                if (strip_plan(nx, ny, ty, halo).total <= cap) {
+                    out[n++] = ty;
+                    break;
+                }
+        }
+    return n;
+}
+
 // The segment of lane `lane` in wavefront `strip`: a full strip is one segment of 64 lanes;
 // a wavefront of the packed remainder strip holds K segments of sp.seg lanes, each with its
 // own rows (segments past the last row get y0 = y1), and the 64 - K*seg lanes after them form
@@ -203,6 +224,10 @@ hipError_t launch_batch_step_control(StepCtl *ctl, int n, hipStream_t s);
 hipError_t launch_batch_dt(const BatchInst *inst, int n, int max_cells, hipStream_t s);
 hipError_t launch_dt(const DtArgs &A, hipStream_t s);
 hipError_t launch_step_control(StepCtl *S, hipStream_t s);
+// dst[0..n) = src[0..n) unless *active == 0 (active may be null: always)
+hipError_t launch_copy_state(double *dst, const double *src, size_t n, const int *active, hipStream_t s);
+// register 1 -> register 0 of every batch instance whose time loop is active
+hipError_t launch_batch_copy_state(const BatchInst *inst, int n, int max_cells, hipStream_t s);
 int strip_width(int scheme, int recon);  // owned columns per wavefront
 int strip_halo(int scheme, int recon);   // halo columns on each side of a strip
 

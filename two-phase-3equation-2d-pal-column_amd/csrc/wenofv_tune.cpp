@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "wenofv.h"
+#include "wenofv_geometry.h"
 #include "wenofv_host.h"
 #include "wenofv_kernels.h"
 
@@ -35,25 +36,6 @@ std::vector<int> row_candidates(int ny) {
         if (r < ny) c.push_back(r);
     c.push_back(std::min(ny, 128));
     c.erase(std::unique(c.begin(), c.end()), c.end());
-    return c;
-}
-
-// A grid of a few hundred thousand to a few million cells gives fewer wavefronts than the
-// device holds at once, or a last round that is nearly empty: a launch lasts as long as
-// its most loaded SIMD.  Added candidates: for 1 and 2 resident wavefronts per SIMD and 1-4
-// rounds, the fewest rows per strip whose wavefront count fits that many rounds exactly.
-std::vector<int> occupancy_candidates(int nx, int ny, int halo, int cus) {
-    std::vector<int> c;
-    const int top = std::min(ny, 128);
-    for (int per_cu : {4, 8})
-        for (int rounds = 1; rounds <= 4; ++rounds) {
-            const long long cap = (long long)rounds * per_cu * cus;
-            for (int ty = 1; ty <= top; ++ty)
-                if (strip_plan(nx, ny, ty, halo).total <= cap) {
-                    c.push_back(ty);
-                    break;
-                }
-        }
     return c;
 }
 
@@ -87,6 +69,9 @@ hipError_t time_launches(hipStream_t s, int trials, L launch, float *ms) {
 
 hipError_t time_ctx_stage(wfv_ctx *c, int trials, float *ms) {
     StageArgs A = base_args(c);
+    // the timed stages leave the recorded limiter flags and the error bits alone
+    A.theta_flag = nullptr;
+    A.flags = nullptr;
     A.U = c->reg[0];
     A.Un = nullptr;
     A.Uo = c->reg[1];
@@ -125,7 +110,8 @@ extern "C" {
 int wfv_set_rows_per_strip(wfv_ctx *c, int rows) {
     if (!c || rows < 0) return WFV_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->ty = rows > 0 ? rows : c->ty;
+    // 0: the default; strips longer than the grid are the whole grid
+    c->ty = rows > 0 ? std::min(rows, c->ny) : default_rows_per_strip(c->nx, c->ny);
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -151,8 +137,9 @@ int wfv_autotune(wfv_ctx *c, int trials, wfv_tuning *best) {
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
     std::vector<int> cand = row_candidates(c->ny);
-    for (int ty : occupancy_candidates(c->nx, c->ny, strip_halo(c->opt.scheme, c->opt.reconstruction), cus))
-        cand.push_back(ty);
+    int occ[8];
+    const int nocc = occupancy_candidates(c->nx, c->ny, strip_halo(c->opt.scheme, c->opt.reconstruction), cus, occ);
+    cand.insert(cand.end(), occ, occ + nocc);
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     for (int ty : cand) {

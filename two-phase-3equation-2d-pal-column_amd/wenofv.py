@@ -104,6 +104,8 @@ def lib():
     L.wfv_set_ghost_cells.argtypes = [vp, _dp]
     L.wfv_set_rows_per_strip.argtypes = [vp, ctypes.c_int]
     L.wfv_get_rows_per_strip.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.wfv_record_limiter_flags.argtypes = [vp, ctypes.c_int]
+    L.wfv_get_limiter_flags.argtypes = [vp, vp]
     L.wfv_version.restype = ctypes.c_char_p
     _lib = L
     return L
