@@ -199,6 +199,28 @@ int wfv_reference_initial_state(int testcase, int nx, int ny, int geometry_ranks
    same spacing): Upad [4][ny+2*WFV_GHOST][nx+2*WFV_GHOST], the ghost cells of boundary 3. */
 int wfv_reference_initial_state_padded(int testcase, int nx, int ny, int geometry_ranks, double *Upad);
 
+/* ---------------------------------------------------------------------------------------
+   Allocation-free entry point.  No counterpart in the reference: it serves callers that
+   cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+   forbid hipMalloc).  The fixed-step form of wfv_advance (fixed_steps unclipped CFL steps
+   from t_start) on a caller-owned device state dU, [4][ny][nx], which is input and output;
+   the grid (opt->elem_num_x x opt->elem_num_y, geometry_ranks), options and case are those
+   wfv_create takes.  The RK registers 1 and 2, the grid widths and the loop control live in
+   a static scratch array of WFV_NOALLOC_SCRATCH_BYTES on each GPU the library runs on (held
+   until the process ends): a grid needs 64 nx ny + 16 nx + 8 ny bytes plus at most 1 KiB, so
+   grids up to 1024 x 1024 cells fit; larger ones are refused (WFV_ERR_ARG), as is a
+   fixed-ghost boundary (3: WFV_ERR_CONFIG).  Every call on a GPU shares that GPU's scratch,
+   so calls that overlap on one GPU race.  The work is enqueued on hip_stream (a hipStream_t;
+   NULL: the null stream).  With nsteps and t_out both NULL the call only enqueues kernels,
+   with no allocation, copy or synchronization (so it can be captured into a graph); else it
+   waits for them, returns the step count and final time, and WFV_ERR_NONFINITE_DT if a CFL
+   time step was NaN.  Failures are described by wfv_last_error(NULL).  Results are
+   bit-identical to wfv_advance(ctx, t_start, 0, fixed_steps, ...) on a context created with
+   the same options and case and given the same state. */
+#define WFV_NOALLOC_SCRATCH_BYTES (65u << 20)
+int wfv_advance_fixed_noalloc(const wfv_options *opt, const wfv_case *cs, double *dU, double t_start,
+                              int fixed_steps, void *hip_stream, int *nsteps, double *t_out);
+
 /* Library build identification (kernel variants compiled for gfx950). */
 const char *wfv_version(void);
 

@@ -106,6 +106,8 @@ def lib():
     L.wfv_get_rows_per_strip.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.wfv_record_limiter_flags.argtypes = [vp, ctypes.c_int]
     L.wfv_get_limiter_flags.argtypes = [vp, vp]
+    L.wfv_advance_fixed_noalloc.argtypes = [ctypes.POINTER(Options), ctypes.POINTER(Case), vp, ctypes.c_double,
+                                            ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int), _dp]
     L.wfv_version.restype = ctypes.c_char_p
     _lib = L
     return L
