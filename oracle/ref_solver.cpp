@@ -11,6 +11,12 @@
 //   theta -- evaluates getPositivityPreservingTheta on cells built from a file.
 // Outputs are raw little-endian doubles written by rank 0.  Run under `mpirun -np 4`
 // from a scratch directory: initializeSolver writes ./output/ there.
+//
+// An optional last argument names a test case whose source term assembleSourceTerm applies
+// (src/WENOFV.cpp:1831-1872 keys on the public m_testcase alone): the solver is initialised
+// as the periodic `testcase` (the gravity cases' own NEUMANN boundary stops the reference's
+// ghost fill, src/WENOFV.cpp:2287-2292, already inside initializeSolver) and m_testcase is
+// then set to it, so DAMBREAK, RTI or STANDING_WAVE gravity runs on a periodic grid.
 #include <mpi.h>
 
 #include <chrono>
@@ -95,7 +101,7 @@ std::vector<double> to_soa(const std::vector<double> &cells, size_t nc) {
 int main(int argc, char **argv) {
     MPI_Init(&argc, &argv);
     // argv: mode testcase nx ny rk cfl scheme pp flux recon t_end|nsteps state_in(or -) out_prefix
-    //       [rho10 rho20 sound1 sound2 preb]
+    //       [rho10 rho20 sound1 sound2 preb [source_testcase]]
     if (argc < 14) {
         std::fprintf(stderr, "usage: refsolver mode testcase nx ny rk cfl scheme pp flux recon "
                              "tend_or_steps state_in out_prefix [rho10 rho20 c1 c2 preb]\n");
@@ -126,6 +132,7 @@ int main(int argc, char **argv) {
         equation->g_sound2 = std::atof(argv[17]);
         equation->g_preb = std::atof(argv[18]);
     }
+    if (argc >= 20) solver.m_testcase = (TESTCASETYPE)std::atoi(argv[19]);
     const int nx = solver.m_worldElemNumX, ny = solver.m_worldElemNumY;
     const size_t nc = (size_t)nx * ny;
     if (state_in != "-" && mode != "theta") {

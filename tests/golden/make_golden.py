@@ -481,6 +481,40 @@ def make_settings():
           f"{100 * 100 * 3 * steps / secs / 1e6:.3f} M cell-stage/s (4 MPI ranks)")
 
 
+# gravity cases of assembleSourceTerm (src/WENOFV.cpp:1831-1872): DAMBREAK and RTI
+# (g = 9.81), STANDING_WAVE (2 pi), applied on the periodic SMOOTH grid (refsolver's
+# source_testcase argument)
+GRAVITY = [(3, "dambreak"), (6, "rti"), (7, "standing_wave")]
+
+
+def make_gravity(rng):
+    out = {}
+    params = (1.2, 1.0, 1.0, 1.0, 0.0)
+    out["params"] = np.array(params)
+    nx = ny = 12
+    U0 = random_state(rng, nx, ny, params, "smooth")
+    out["rhs_U"] = U0
+    for tc, name in GRAVITY:
+        for scheme, recon, flux, bp in ((1, 0, 1, 1), (0, 1, 0, 0)):
+            tag = f"rhs_{name}_s{scheme}_r{recon}_f{flux}_b{bp}"
+            with tempfile.TemporaryDirectory() as tmp:
+                prefix, _ = run_ref(["rhs", 0, nx, ny, 2, 0.4, scheme, bp, flux, recon, 0.0, *params, tc], tmp,
+                                    state=U0)
+                assert np.array_equal(load(prefix, "_init.bin", (4, ny, nx)), U0)
+                out[tag] = load(prefix, "_rhs.bin", (4, ny, nx))
+    nx = ny = 24
+    U0 = random_state(rng, nx, ny, params, "smooth")
+    out["run_U"] = U0
+    for tc, name in GRAVITY:
+        with tempfile.TemporaryDirectory() as tmp:
+            prefix, so = run_ref(["steps", 0, nx, ny, 2, 0.4, 1, 1, 1, 0, 20, *params, tc], tmp, state=U0)
+            out[f"run_{name}_final"] = load(prefix, "_final.bin", (4, ny, nx))
+            out[f"run_{name}_dts"] = load(prefix, "_dts.bin")
+            print(name, so.strip().splitlines()[-1])
+    np.savez_compressed(os.path.join(HERE, "solver_gravity.npz"), **out)
+    print("gravity fixtures written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--only", nargs="*", default=None)
@@ -497,7 +531,8 @@ def main():
             "outputs": make_outputs,
             "c1": make_c1,
             "c1char": lambda: make_c1(1, "c1_char_reference.npz"),
-            "settings": make_settings}
+            "settings": make_settings,
+            "gravity": lambda: make_gravity(np.random.default_rng(7009))}
     for k, f in jobs.items():
         if a.only is None or k in a.only:
             f()

@@ -1,10 +1,12 @@
 """Batched solver: every instance of a batch reproduces a single-domain solver with the
 same options bit for bit (tolerance 0), in both precision policies, with uneven grids,
-periodic and wall boundaries and per-instance time steps."""
+periodic and wall boundaries and per-instance time steps; in all-FP64 every instance also
+equals the oracle run on its own (state and final time, bit for bit)."""
 import numpy as np
 import pytest
 
 import cases
+import oracle as orc
 import wenofv
 
 pytestmark = pytest.mark.gpu
@@ -42,6 +44,12 @@ def test_batch_instances_equal_single_solvers(precision, rk, recon):
         ns, ts, _ = sol.advance(fixed_steps=7)
         assert ts == t[k]
         np.testing.assert_array_equal(b.get_state(k), sol.get_state())
+        if precision == 0:
+            o = orc.Oracle(wl.nx, wl.ny, domain=wl.domain, bc=insts[k]["bc"], nranks=1, **wl.params,
+                           **{q: opts[q] for q in ("scheme", "recon", "flux", "bp", "rk", "cfl")})
+            U_cpu, _, t_cpu, err = o.run(wl.U0, fixed_steps=7)
+            assert err == 0 and t[k] == t_cpu
+            np.testing.assert_array_equal(b.get_state(k), U_cpu)
 
 
 def test_batch_time_driven_per_instance_dt():
@@ -59,6 +67,11 @@ def test_batch_time_driven_per_instance_dt():
         ns, ts, _ = sol.advance(t_end=2e-3)
         assert ns == n[k] and ts == t[k]
         np.testing.assert_array_equal(b.get_state(k), sol.get_state())
+        o = orc.Oracle(wl.nx, wl.ny, domain=wl.domain, bc=1, nranks=1, **wl.params,
+                       **{q: opts[q] for q in ("scheme", "recon", "flux", "bp", "rk", "cfl")})
+        U_cpu, dts_cpu, t_cpu, err = o.run(wl.U0, t_end=2e-3)
+        assert err == 0 and len(dts_cpu) == n[k] and t_cpu == t[k]
+        np.testing.assert_array_equal(b.get_state(k), U_cpu)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
