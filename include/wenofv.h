@@ -44,11 +44,12 @@ typedef struct wfv_options {
     int elem_num_y;      /* ELEMNUMY */
     int rk_method;       /* RKMETH: 0 SSP-RK1, 1 SSP-RK2, 2 SSP-RK3 */
     double cfl;          /* CFL */
-    int scheme;          /* SCHEME: 0 WENO-JS, 1 WENO-Z */
+    int scheme;          /* SCHEME: 0 WENO-JS, 1 WENO-Z (5-point stencils); 2 WENO-EXP (7-point) */
     int pp_limiter;      /* PPlimiter: bound-preserving limiter on the partial densities */
     int riemann_flux;    /* RIEMANNFLUX: 0 LLF, 1 HLLC */
     int reconstruction;  /* RECONSTRUCTIONTYPE: 0 conservative, 1 characteristic,
-                            2 primitive (z1, rho1, u, v), 3 primitive2 (z1, p, u, v) */
+                            2 primitive (z1, rho1, u, v), 3 primitive2 (z1, p, u, v),
+                            4 primitive3 (THINC on z1 and WENO-Z on p, u, v; 7-point) */
     int precision;       /* 0: all-FP64, bit-faithful to the reference; 1: mixed precision */
     int boundary;        /* BOUNDARYTYPE (reference include/Equations.hpp:25-32; its MPI solver
                             runs only 0): 0 periodic; 1 slip walls (NEUMANN, FREESLIP: mirror,
@@ -95,6 +96,21 @@ int wfv_step(wfv_ctx *ctx, double dt);
    faces (optional): limited face values per cell [ny][nx][side][var][gp] with side order
    left, right, bottom, top (the reference's m_gridFlux); theta (optional): [ny][nx]. */
 int wfv_assemble_rhs(wfv_ctx *ctx, double *rhs, double *faces, double *theta);
+
+/* CWENOFV::getFlux_conservative / _characteristic / _primitive / _primitive2 / _primitive3
+   (src/WENOFV.cpp:617-1372) of the current state: the face values (the reference's
+   m_gridFlux) of reconstruction form `reconstruction` (RECONSTRUCTIONTYPE 0..4) with the
+   context's scheme, boundary and precision, limited by the bound-preserving limiter
+   (useBoundPreservingLimiter, src/WENOFV.cpp:1895-1935) when limit is 1, for the cells and
+   the ring of ghost cells around them: faces [ny+2][nx+2][side][var][gp] (sides left, right,
+   bottom, top; padded row 0 / column 0 hold the ghost cells below / left of the grid);
+   theta (optional) [ny+2][nx+2] the limiter's theta (1 where limit is 0). */
+int wfv_face_values(wfv_ctx *ctx, int reconstruction, int limit, double *faces, double *theta);
+
+/* CWENOFV::calculateDeltaT's time step of every cell (m_totalDeltaT, src/WENOFV.cpp:489-500)
+   of the current state: dt [ny][nx] = CFL/(ax/dx + ay/dy), NaN where the state gives none
+   (the map the reference writes with outputDeltaT before it stops on a NaN time step). */
+int wfv_cell_dt(wfv_ctx *ctx, double *dt);
 
 /* CWENOFV::run's time loop (src/WENOFV.cpp:318-332) on the device: from t_start, steps of
    dt = min(CFL dt, t_end - now) until |now - t_end| <= 1e-10, or exactly fixed_steps
@@ -198,6 +214,32 @@ int wfv_reference_initial_state(int testcase, int nx, int ny, int geometry_ranks
 /* The same on the grid padded by WFV_GHOST cells on every side (cells extended with the
    same spacing): Upad [4][ny+2*WFV_GHOST][nx+2*WFV_GHOST], the ghost cells of boundary 3. */
 int wfv_reference_initial_state_padded(int testcase, int nx, int ny, int geometry_ranks, double *Upad);
+
+/* ---------------------------------------------------------------------------------------
+   Point functions of the model, for one state U[4] = (z1 rho1, z2 rho2, rho u, rho v) of a
+   case cs (its rho10, rho20, sound1, sound2, preb): the reference's inline TwoPhaseEquation
+   and CWENOFV functions, evaluated by the all-FP64 kernels' own arithmetic (so they round
+   as the reference's do).  Host only, no device work.  A normal (nx, ny) selects the x
+   direction as the reference does: the physical flux where nx > ny, the eigenvectors where
+   |nx| > |ny|; the Riemann fluxes take it as it is. */
+/* TwoPhaseEquation::getVolumeFrac / getPressure / getSoundSpeed (include/Equations.hpp:84-146):
+   out = z1, p, c, rho, rho1, rho2 */
+int wfv_point_eos(const wfv_case *cs, const double *U, double *out);
+/* TwoPhaseEquation::getPhyFlux (include/Equations.hpp:158-186) */
+int wfv_point_flux(const wfv_case *cs, const double *U, double nx, double ny, double *F);
+/* TwoPhaseEquation::getLLFRiemannFlux / getHLLCRiemannFlux (include/Equations.hpp:363-483):
+   riemann_flux 0 LLF, 1 HLLC */
+int wfv_point_riemann(const wfv_case *cs, int riemann_flux, const double *UL, const double *UR, double nx,
+                      double ny, double *F);
+/* TwoPhaseEquation::getLEigenMatrix / getREigenMatrix (include/Equations.hpp:189-327),
+   row-major 4 x 4 */
+int wfv_point_eigen(const wfv_case *cs, const double *U, double nx, double ny, double *L, double *R);
+/* CWENOFV::useWENO (src/WENOFV.cpp:1873-1893) at the 4 Gauss-Lobatto nodes: s[7] the averages
+   u_mmm..u_ppp, scheme 0 WENO-JS and 1 WENO-Z (on s[1..5]), 2 WENO-EXP (dx the cell width) */
+int wfv_point_weno(int scheme, const double *s, double dx, double *out);
+/* CWENOFV::getPositivityPreservingTheta (src/WENOFV.cpp:1936-2000) of a cell: faces[64] its
+   face values [side][var][gp] (sides left, right, bottom, top), dX, dY the global widths */
+int wfv_point_theta(const wfv_case *cs, const double *U, const double *faces, double dX, double dY, double *theta);
 
 /* ---------------------------------------------------------------------------------------
    Allocation-free entry point.  No counterpart in the reference: it serves callers that

@@ -39,7 +39,7 @@ if [ -f "$MPI_INC/mpi.h" ] && [ -f "$MPI_LIB/libmpi.so" ]; then
     mkdir -p "$T/src" "$T/include"
     ln -s "$REF/src/main.cpp" "$T/src/main.cpp"
     ln -s "$REF/third_party" "$T/third_party"
-    for h in WENOFV.hpp ReadConfig.hpp Equations.hpp; do ln -s "$PKG/csrc/$h" "$T/include/$h"; done
+    for h in WENOFV.hpp WENOFV_facade.hpp WENOFV_output.hpp ReadConfig.hpp Equations.hpp wenofv_array.h; do ln -s "$PKG/csrc/$h" "$T/include/$h"; done
     g++ -std=c++17 -O2 -w -I"$HERE/../include" -I"$MPI_INC" -c "$T/src/main.cpp" -o "$OUT/refmain.o"
     g++ -o "$OUT/refmain_gpu" "$OUT/refmain.o" -L"$PKG" -lwenofv "$MPI_LIB/libmpi.so" \
         -Wl,-rpath,'$ORIGIN/../../two-phase-3equation-2d-pal-column_amd':/usr/lib/x86_64-linux-gnu:"$MPI_LIB"

@@ -30,8 +30,8 @@ UNITS = [
     ("wenofv_noalloc.hip", ["-ffp-contract=off"]),
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),
     ("wenofv_tune.cpp", ["-ffp-contract=off"]),
-    ("WENOFV.cpp", ["-ffp-contract=off"]),
-    ("WENOFV_output.cpp", ["-ffp-contract=off"]),
+    ("wenofv_point.cpp", ["-ffp-contract=off"]),
+    ("wenofv_cases.cpp", ["-ffp-contract=off"]),
 ]
 CLI = os.path.join(HERE, "wenofv_run")
 
@@ -53,12 +53,17 @@ def build(verbose=False, out=LIB, extra_flags=()):
     if failed:
         raise RuntimeError(f"compilation failed: {failed}")
     tmp = out + ".tmp"
-    subprocess.check_call([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    # only the C ABI (wfv_*) is exported: the C++ facade is header-only and compiles in the
+    # caller's translation unit
+    subprocess.check_call([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs +
+                          ["-Wl,--version-script=" + os.path.join(CSRC, "wenofv.map")])
     os.replace(tmp, out)
     if out == LIB:
-        # command-line driver (config/settings.cfg), linked against the library in place
-        subprocess.check_call([HIPCC] + COMMON + ["-ffp-contract=off", os.path.join(CSRC, "main.cpp"), "-o", CLI + ".tmp",
-                               "-L" + HERE, "-lwenofv", "-Wl,-rpath,$ORIGIN"])
+        # command-line driver (config/settings.cfg): the facade headers compiled by the host
+        # compiler, without contraction as the reference's build, linked against the library
+        subprocess.check_call(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-I" + os.path.join(REPO, "include"),
+                               "-I" + CSRC, os.path.join(CSRC, "main.cpp"), "-o", CLI + ".tmp", "-L" + HERE,
+                               "-lwenofv", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"])
         os.replace(CLI + ".tmp", CLI)
     return out
 

@@ -1,16 +1,22 @@
 // Equations.hpp -- host-side model of the two-phase three-equation system: test cases,
-// initial conditions and the equation of state, with the public surface of the
+// initial conditions, and the point functions of the model (equation of state, physical and
+// Riemann fluxes, eigenvectors, derived output variables), with the public surface of the
 // reference's TwoPhaseEquation (include/Equations.hpp:52-1044 of the reference), so that
 // code written against that class compiles against this one.
 //
-// The time-stepping arithmetic itself runs in the HIP kernels (wenofv_device.h); this
-// class only feeds them parameters and initial cell averages.
+// Header-only, so it compiles with the caller's containers (wenofv_array.h).  The point
+// functions call the library's wfv_point_* entry points, which evaluate the kernels' own
+// all-FP64 functions: a value computed here is the value a kernel computes.
 #pragma once
 #include <cmath>
 #include <functional>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include "wenofv.h"
+#include "wenofv_array.h"
 
 enum RIEMANNFLUXTYPE { LLF = 0, HLLC = 1 };
 
@@ -40,7 +46,7 @@ public:
     double xL = 0, xR = 1, yL = 0, yR = 1;
     std::vector<double> outputTimes;
     BOUNDARYTYPE boundaryCondition = PERIOD;
-    std::function<double(const double *)> theVarUh;        // scalar checked by the accuracy output
+    std::function<double(Array1D<double>)> theVarUh;       // scalar checked by the accuracy output
     std::function<double(double, double, double)> theVarExact;
     bool u_exact_exist = false;
 
@@ -54,21 +60,126 @@ public:
 
     int getVarNum() const { return 4; }
 
-    // TwoPhaseEquation::getVolumeFrac (include/Equations.hpp:84-99)
-    double getVolumeFrac(const double *U) const {
-        const double c1_2 = g_sound1 * g_sound1, c2_2 = g_sound2 * g_sound2;
-        const double q = g_rho20 * c2_2 - g_rho10 * c1_2;
-        const double qs = U[1] * c2_2 - U[0] * c1_2;
-        const double r = ((q - qs) + std::sqrt((q - qs) * (q - qs) + 4.0 * U[0] * c1_2 * U[1] * c2_2)) /
-                         (2.0 * U[1] * c2_2);
-        return r / (1.0 + r);
+    // ------------------------------------------------------------ point functions
+    // U = (z1 rho1, z2 rho2, rho u, rho v).  A normal (nx, ny) selects the x direction where
+    // the reference does: getPhyFlux where nx > ny, the eigenvector matrices where
+    // |nx| > |ny|; the Riemann fluxes and the maximum eigenvalue take it as it is.
+
+    // the phase parameters as the library's point functions take them
+    wfv_case pointCase() const {
+        wfv_case c{};
+        c.xL = xL;
+        c.xR = xR;
+        c.yL = yL;
+        c.yR = yR;
+        c.rho10 = g_rho10;
+        c.rho20 = g_rho20;
+        c.sound1 = g_sound1;
+        c.sound2 = g_sound2;
+        c.preb = g_preb;
+        c.gravity = g_gravity;
+        return c;
     }
-    // TwoPhaseEquation::getPressure (include/Equations.hpp:101-124)
+    // z1, p, c, rho, rho1, rho2 of a state
+    void getEos(const double *U, double e[6]) const {
+        const wfv_case c = pointCase();
+        if (wfv_point_eos(&c, U, e) != WFV_OK) throw std::invalid_argument("getEos");
+    }
+
+    // getVolumeFrac (include/Equations.hpp:84-99)
+    double getVolumeFrac(const Array1D<double> U) const { return getVolumeFrac(&U[0]); }
+    double getVolumeFrac(const double *U) const {
+        double e[6];
+        getEos(U, e);
+        return e[0];
+    }
+    // getPressure (include/Equations.hpp:101-124)
+    double getPressure(const Array1D<double> U) const { return getPressure(&U[0]); }
     double getPressure(const double *U) const {
-        const double z1 = getVolumeFrac(U), z2 = 1.0 - z1;
-        const double pre1 = g_sound1 * g_sound1 * (U[0] / z1 - g_rho10);
-        const double pre2 = g_sound2 * g_sound2 * (U[1] / z2 - g_rho20);
-        return z1 < z2 ? pre2 : pre1;
+        double e[6];
+        getEos(U, e);
+        return e[1];
+    }
+    // getSoundSpeed (include/Equations.hpp:126-146): Wood's mixture sound speed
+    double getSoundSpeed(const Array1D<double> U) const {
+        double e[6];
+        getEos(&U[0], e);
+        return e[2];
+    }
+    // getMaxEigenValue (include/Equations.hpp:149-156): |u.n| + c
+    double getMaxEigenValue(const Array1D<double> &U, double nx, double ny) const {
+        const double c = getSoundSpeed(U);
+        const double u = U[2] / (U[0] + U[1]);
+        const double v = U[3] / (U[0] + U[1]);
+        return std::fabs(u * nx + v * ny) + c;
+    }
+    // getPhyFlux (include/Equations.hpp:158-186)
+    void getPhyFlux(const Array1D<double> Uh, Array1D<double> &Flux, double nx, double ny) const {
+        if (Flux.dim() < 4) Flux.Resize(4);
+        const wfv_case c = pointCase();
+        wfv_point_flux(&c, &Uh[0], nx, ny, &Flux[0]);
+    }
+    // getREigenMatrix / getLEigenMatrix (include/Equations.hpp:189-327), 4 x 4
+    void getREigenMatrix(const Array1D<double> &U, double nx, double ny, Array2D<double> &eigMatrix) const {
+        eigen(U, nx, ny, eigMatrix, false);
+    }
+    void getLEigenMatrix(const Array1D<double> &U, double nx, double ny, Array2D<double> &eigMatrix) const {
+        eigen(U, nx, ny, eigMatrix, true);
+    }
+    // getRiemannFlux (include/Equations.hpp:329-360): LLF (riemannFluxType 0) or HLLC (1); a
+    // NaN flux is reported on std::cerr, as the reference does (without its pause on stdin)
+    void getRiemannFlux(const Array1D<double> UL, const Array1D<double> UR, Array1D<double> &Flux, double nx, double ny,
+                        int riemannFluxType) const {
+        if (riemannFluxType != LLF && riemannFluxType != HLLC) {
+            std::cerr << "Riemann flux type is not supported." << std::endl;
+            return;
+        }
+        if (Flux.dim() < 4) Flux.Resize(4);
+        const wfv_case c = pointCase();
+        wfv_point_riemann(&c, riemannFluxType, &UL[0], &UR[0], nx, ny, &Flux[0]);
+        for (int r = 0; r != getVarNum(); ++r)
+            if (std::isnan(Flux[r])) {
+                std::cerr << "Riemann flux is nan." << std::endl;
+                std::cerr << "UL: " << UL << std::endl << "UR: " << UR << std::endl;
+                std::cerr << "nx: " << nx << std::endl << "ny: " << ny << std::endl;
+                std::cerr << "riemannFluxType: " << riemannFluxType << std::endl << "r: " << r << std::endl;
+                std::cerr << "Flux: " << Flux << std::endl;
+            }
+    }
+    // getLLFRiemannFlux (include/Equations.hpp:363-378), getHLLCRiemannFlux (381-483)
+    void getLLFRiemannFlux(const Array1D<double> UL, const Array1D<double> UR, Array1D<double> &Flux, double nx,
+                           double ny) const {
+        if (Flux.dim() < 4) Flux.Resize(4);
+        const wfv_case c = pointCase();
+        wfv_point_riemann(&c, LLF, &UL[0], &UR[0], nx, ny, &Flux[0]);
+    }
+    void getHLLCRiemannFlux(const Array1D<double> &UL, const Array1D<double> &UR, Array1D<double> &Flux, double nx,
+                            double ny) const {
+        if (Flux.dim() < 4) Flux.Resize(4);
+        const wfv_case c = pointCase();
+        wfv_point_riemann(&c, HLLC, &UL[0], &UR[0], nx, ny, &Flux[0]);
+    }
+    // getVitalVarVal / Num / Name (include/Equations.hpp:1005-1043): the 9 output variables
+    // (z1 rho1, z2 rho2, z1, rho1, rho2, rho, u, v, p)
+    void getVitalVarVal(const Array1D<double> Uh, Array1D<double> &VitalVar) const {
+        double e[6];
+        getEos(&Uh[0], e);
+        if (VitalVar.dim() < 9) VitalVar.Resize(9);
+        VitalVar[0] = Uh[0];
+        VitalVar[1] = Uh[1];
+        VitalVar[2] = e[0];
+        VitalVar[3] = e[4];
+        VitalVar[4] = e[5];
+        VitalVar[5] = Uh[0] + Uh[1];
+        VitalVar[6] = Uh[2] / (Uh[0] + Uh[1]);
+        VitalVar[7] = Uh[3] / (Uh[0] + Uh[1]);
+        VitalVar[8] = e[1];
+    }
+    int getVitalVarNum() const { return 9; }
+    void getVitalVarName(Array1D<std::string> &VitalVarName) const {
+        static const char *names[9] = {"RHO1Z1", "RHO2Z2", "Z1", "RHO1", "RHO2", "RHO", "U", "V", "PRE"};
+        if (VitalVarName.dim() < 9) VitalVarName.Resize(9);
+        for (int k = 0; k < 9; ++k) VitalVarName[k] = names[k];
     }
 
     // TwoPhaseEquation::setEquationParameters (include/Equations.hpp:486-977): domain,
@@ -92,7 +203,7 @@ public:
             uniform();
             u_exact_exist = true;
             theVarExact = [this](double x, double y, double t) { return z1_0(x - g_u0 * t, y - g_v0 * t); };
-            theVarUh = [this](const double *U) { return getVolumeFrac(U); };
+            theVarUh = [this](Array1D<double> U) { return getVolumeFrac(U); };
             break;
         case VORTEX: {
             xL = -1; xR = 1; yL = -1; yR = 1;
@@ -128,7 +239,7 @@ public:
             };
             u_exact_exist = true;
             theVarExact = [this](double x, double y, double t) { return z1_0(x - g_u0 * t, y - g_v0 * t); };
-            theVarUh = [this](const double *U) { return getVolumeFrac(U); };
+            theVarUh = [this](Array1D<double> U) { return getVolumeFrac(U); };
             break;
         }
         case SHARP:
@@ -149,7 +260,7 @@ public:
                 const double z1 = (std::fabs(x - xc) < r && std::fabs(y - yc) < r) ? 1 - g_epsa : g_epsa;
                 return z1 * g_rho10;
             };
-            theVarUh = [](const double *U) { return U[0]; };
+            theVarUh = [](Array1D<double> U) { return U[0]; };
             break;
         case ZALESAK:
             xL = 0; xR = 1; yL = 0; yR = 1;
@@ -165,7 +276,7 @@ public:
             v_0 = [](double x, double) { return M_PI * (x - 0.5); };
             u_exact_exist = false;
             theVarExact = [](double, double, double) { return 0.0; };
-            theVarUh = [](const double *U) { return U[0]; };
+            theVarUh = [](Array1D<double> U) { return U[0]; };
             break;
         case STANDING_WAVE:
             xL = 0; xR = 0.5; yL = 0; yR = 0.5;
@@ -181,7 +292,7 @@ public:
             uniform();
             u_exact_exist = false;
             theVarExact = [](double, double, double) { return 0.0; };
-            theVarUh = [](const double *U) { return U[0]; };
+            theVarUh = [](Array1D<double> U) { return U[0]; };
             break;
         case RTI:
             xL = 0; xR = 1; yL = 0; yR = 4;
@@ -197,7 +308,7 @@ public:
             uniform();
             u_exact_exist = false;
             theVarExact = [](double, double, double) { return 0.0; };
-            theVarUh = [](const double *U) { return U[0]; };
+            theVarUh = [](Array1D<double> U) { return U[0]; };
             break;
         case DAMBREAK:
         case DAMBREAK2:
@@ -221,7 +332,7 @@ public:
             uniform();
             u_exact_exist = false;
             theVarExact = [](double, double, double) { return 0.0; };
-            theVarUh = [](const double *U) { return U[0]; };
+            theVarUh = [](Array1D<double> U) { return U[0]; };
             break;
         }
         default:
@@ -230,6 +341,10 @@ public:
     }
 
     // TwoPhaseEquation::getU0 (include/Equations.hpp:979-1002)
+    void getU0(double xP, double yP, Array1D<double> &U) const {
+        if (U.dim() < 4) U.Resize(4);
+        getU0(xP, yP, &U[0]);
+    }
     void getU0(double xP, double yP, double *U) const {
         const double z_1 = z1_0(xP, yP), z_2 = 1.0 - z_1;
         const double p_1 = p1_0(xP, yP), p_2 = p2_0(xP, yP);
@@ -242,5 +357,15 @@ public:
         U[1] = z2rho2;
         U[2] = rho * u;
         U[3] = rho * v;
+    }
+
+private:
+    void eigen(const Array1D<double> &U, double nx, double ny, Array2D<double> &M, bool left) const {
+        double L[16], R[16];
+        const wfv_case c = pointCase();
+        wfv_point_eigen(&c, &U[0], nx, ny, L, R);
+        if (M.row() < 4 || M.col() < 4) M.Resize(4, 4);
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) M[i][j] = left ? L[i * 4 + j] : R[i * 4 + j];
     }
 };

@@ -773,8 +773,23 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     err |= WFV_FLAG_BAD_THETA;
                 }
             }
-            if (A.faces_out && owned && r >= y0 && r < y1) {
-                double *fo = A.faces_out + ((size_t)r * nx + xc) * 64;
+            // parity hook: limited face values (and theta) of the owned cells; with faces_ring
+            // also of the ring of ghost cells around the grid, in a layout padded by one cell
+            bool fwrite = false;
+            size_t fcell = 0;
+            if (A.faces_out) {
+                if (A.faces_ring) {
+                    const bool col = (j >= HALO && j < HALO + SW && xc < nx) || xc == -1 || xc == nx;
+                    const bool row = (r >= y0 && r < y1) || (r == -1 && y0 == 0) || (r == ny && y1 == ny);
+                    fwrite = col && row && nrow > 0;
+                    fcell = (size_t)(r + 1) * (nx + 2) + (xc + 1);
+                } else {
+                    fwrite = owned && r >= y0 && r < y1;
+                    fcell = (size_t)r * nx + xc;
+                }
+            }
+            if (fwrite) {
+                double *fo = A.faces_out + fcell * 64;
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -784,7 +799,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                         fo[2 * 16 + v * 4 + g] = fs.get(v, g, Uc[v]);
                         fo[3 * 16 + v * 4 + g] = fn.get(v, g, Uc[v]);
                     }
-                if (A.theta_out) A.theta_out[(size_t)r * nx + xc] = theta;
+                if (A.theta_out) A.theta_out[fcell] = theta;
             }
         }
 
@@ -963,6 +978,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.flags = &ctl->flags;
     A.dtmin = B.want_dtmin ? &ctl->dtmin_bits : nullptr;
     A.faces_out = nullptr;
+    A.faces_ring = 0;
     A.theta_out = nullptr;
     A.theta_flag = nullptr;
     stage_body<PR, RECON, SCHEME, FLUX, PF>(A, item - I.item_begin, L);

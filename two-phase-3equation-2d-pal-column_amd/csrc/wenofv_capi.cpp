@@ -234,6 +234,9 @@ void wfv_destroy(wfv_ctx *c) {
     (void)hipFree(c->d_dyC);
     (void)hipFree(c->d_faces);
     (void)hipFree(c->d_theta);
+    (void)hipFree(c->d_faces_ring);
+    (void)hipFree(c->d_theta_ring);
+    (void)hipFree(c->d_cell_dt);
     (void)hipFree(c->d_ghost);
     (void)hipFree(c->d_theta_flag);
     (void)hipFree(c->d_ctl);
@@ -323,6 +326,46 @@ int wfv_assemble_rhs(wfv_ctx *c, double *rhs, double *faces, double *theta) {
     HIPCHK(c, hipMemcpy(rhs, c->reg[2], 4 * c->nc * sizeof(double), hipMemcpyDeviceToHost));
     if (faces) HIPCHK(c, hipMemcpy(faces, c->d_faces, c->nc * 64 * sizeof(double), hipMemcpyDeviceToHost));
     if (theta) HIPCHK(c, hipMemcpy(theta, c->d_theta, c->nc * sizeof(double), hipMemcpyDeviceToHost));
+    return WFV_OK;
+}
+
+int wfv_face_values(wfv_ctx *c, int reconstruction, int limit, double *faces, double *theta) {
+    if (!c || !faces) return WFV_ERR_ARG;
+    if (reconstruction < 0 || reconstruction > 4 || limit < 0 || limit > 1)
+        return fail(c, WFV_ERR_ARG, "wfv_face_values: reconstruction 0..4, limit 0 or 1");
+    const size_t np = (size_t)(c->nx + 2) * (c->ny + 2);
+    if (!c->d_faces_ring) {
+        HIPCHK(c, hipMalloc(&c->d_faces_ring, np * 64 * sizeof(double)));
+        HIPCHK(c, hipMalloc(&c->d_theta_ring, np * sizeof(double)));
+    }
+    wfv_options o = c->opt;
+    o.reconstruction = reconstruction;
+    o.pp_limiter = limit;
+    StageArgs A = stage_base(o, c->cs, c->nx, c->ny, c->ty, c->d_dxC, c->d_dxW, c->d_dyC, c->d_ghost, c->d_ctl);
+    A.U = c->reg[0];
+    A.Un = nullptr;
+    A.Uo = c->reg[2];  // the right-hand side, unused
+    A.a = 0.0;
+    A.b = 0.0;
+    A.c = 1.0;
+    A.dt = 1.0;
+    A.faces_out = c->d_faces_ring;
+    A.theta_out = c->d_theta_ring;
+    A.faces_ring = 1;
+    HIPCHK(c, launch_stage(A, c->opt.precision, reconstruction, c->opt.scheme, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(faces, c->d_faces_ring, np * 64 * sizeof(double), hipMemcpyDeviceToHost));
+    if (theta) HIPCHK(c, hipMemcpy(theta, c->d_theta_ring, np * sizeof(double), hipMemcpyDeviceToHost));
+    return WFV_OK;
+}
+
+int wfv_cell_dt(wfv_ctx *c, double *dt) {
+    if (!c || !dt) return WFV_ERR_ARG;
+    if (!c->d_cell_dt) HIPCHK(c, hipMalloc(&c->d_cell_dt, c->nc * sizeof(double)));
+    DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, nullptr, nullptr, c->d_cell_dt};
+    HIPCHK(c, launch_dt(D, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(dt, c->d_cell_dt, c->nc * sizeof(double), hipMemcpyDeviceToHost));
     return WFV_OK;
 }
 

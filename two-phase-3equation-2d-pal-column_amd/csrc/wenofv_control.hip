@@ -25,6 +25,7 @@ __global__ __launch_bounds__(DT_THREADS) void dt_kernel(DtArgs A) {
         const double ex = fabs(u * 1.0 + v * 0.0) + e.c;
         const double ey = fabs(u * 0.0 + v * 1.0) + e.c;
         const double tmp = A.cfl / (ex / A.dX + ey / A.dY);
+        if (A.cell_dt) A.cell_dt[c] = tmp;  // the reference's m_totalDeltaT (src/WENOFV.cpp:500)
         if (isnan(tmp)) err |= WFV_FLAG_NAN_DT;
         else dtloc = fmin(dtloc, tmp);
     }
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(DT_THREADS) void dt_kernel(DtArgs A) {
             err |= serr[k];
         }
         if (err && A.flags) atomicOr(A.flags, err);
-        if (dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
+        if (A.dtmin && dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
     }
 }
 

@@ -18,6 +18,10 @@
 #include "weno_constants.h"
 
 #define WFV_DEV __device__ __forceinline__
+// the all-FP64 policy's point functions, also compiled for the host: the CWENOFV and
+// TwoPhaseEquation methods that evaluate single points (equation of state, fluxes,
+// eigenvectors, WENO, the limiter's theta) call the very code the kernels run
+#define WFV_HD __host__ __device__ __forceinline__
 // global-memory view of a device pointer: accesses through it compile to global_* rather
 // than flat_* instructions, which wait on the vector-memory counter only (a flat access
 // also holds up the wavefront's next wait for its LDS operations)
@@ -47,9 +51,9 @@ struct Phys {
 constexpr double GLW[4] = {0.166666666666666666666666666667, 0.833333333333333333333333333333,
                                                0.833333333333333333333333333333, 0.166666666666666666666666666667};
 
-WFV_DEV double stdMin(double a, double b) { return (b < a) ? b : a; }  // std::min
-WFV_DEV double stdMax(double a, double b) { return (a < b) ? b : a; }  // std::max
-WFV_DEV double scMin(double a, double b) { return a < b ? a : b; }      // sc_math::Min
+WFV_HD double stdMin(double a, double b) { return (b < a) ? b : a; }  // std::min
+WFV_HD double stdMax(double a, double b) { return (a < b) ? b : a; }  // std::max
+WFV_HD double scMin(double a, double b) { return a < b ? a : b; }      // sc_math::Min
 
 // ------------------------------------------------------------------ equation of state
 // TwoPhaseEquation::getVolumeFrac / getPressure / getSoundSpeed (include/Equations.hpp:84-146)
@@ -57,7 +61,7 @@ WFV_DEV double scMin(double a, double b) { return a < b ? a : b; }      // sc_ma
 struct Eos {
     double rho, z1, rho1, rho2, p, c;
 };
-WFV_DEV void eos(const Phys &P, const double U[4], Eos &e) {
+WFV_HD void eos(const Phys &P, const double U[4], Eos &e) {
     const double z1rho1 = U[0], z2rho2 = U[1];
     const double qs = z2rho2 * P.c2_2 - z1rho1 * P.c1_2;
     const double dq = P.q - qs;
@@ -131,27 +135,26 @@ WFV_DEV void eos_p(const Phys &P, const double U[4], Eos &e) {
 }
 
 // ------------------------------------------------------------------ Riemann fluxes
-// F = face-normal flux through (NX, NY) in {(1,0), (0,1)}.  The literal products with
-// the normal's components are kept so that the double policy matches the reference's
-// rounding and signed zeros.
-template <class T, int NX, int NY>
-WFV_DEV void phy_flux(const T U[4], T rho, T u, T v, T p, T F[4]) {
+// F = the physical flux along x (xdir: the reference's nx > ny) or y.
+template <class T>
+WFV_HD void phy_flux(const T U[4], T rho, T u, T v, T p, bool xdir, T F[4]) {
     // TwoPhaseEquation::getPhyFlux (include/Equations.hpp:158-186)
-    if constexpr (NX > NY) {
+    if (xdir) {
         F[0] = U[0] * u; F[1] = U[1] * u; F[2] = rho * (u * u) + p; F[3] = rho * u * v;
     } else {
         F[0] = U[0] * v; F[1] = U[1] * v; F[2] = rho * v * u; F[3] = rho * (v * v) + p;
     }
 }
 
-// The all-FP64 policy's Riemann solvers: the reference's formulas, operation by operation.
-template <int NX, int NY>
-WFV_DEV void riemann_llf(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
+// The all-FP64 policy's Riemann solvers: the reference's formulas, operation by operation,
+// through the face normal (nx, ny).  The kernels pass (1, 0) or (0, 1) as constants (the
+// riemann_* templates below); the literal products with the normal's components are kept,
+// so that the rounding and signed zeros match the reference's.
+WFV_HD void riemann_llf_n(const Phys &P, const double UL[4], const double UR[4], double nx, double ny, double F[4]) {
     // getLLFRiemannFlux (include/Equations.hpp:363-378)
     Eos el, er;
     eos(P, UL, el);
     eos(P, UR, er);
-    const double nx = NX, ny = NY;
     const double rl = el.rho, rr = er.rho;
     const double ul = UL[2] / rl, vl = UL[3] / rl;
     const double ur = UR[2] / rr, vr = UR[3] / rr;
@@ -159,19 +162,17 @@ WFV_DEV void riemann_llf(const Phys &P, const double UL[4], const double UR[4], 
     const double wsR = fabs(ur * nx + vr * ny) + er.c;
     const double ws = stdMax(wsL, wsR);
     double FL[4], FR[4];
-    phy_flux<double, NX, NY>(UL, rl, ul, vl, el.p, FL);
-    phy_flux<double, NX, NY>(UR, rr, ur, vr, er.p, FR);
+    phy_flux<double>(UL, rl, ul, vl, el.p, nx > ny, FL);
+    phy_flux<double>(UR, rr, ur, vr, er.p, nx > ny, FR);
 #pragma unroll
     for (int r = 0; r < 4; ++r) F[r] = 0.5 * (FL[r] + FR[r] - ws * (UR[r] - UL[r]));
 }
 
-template <int NX, int NY>
-WFV_DEV void riemann_hllc(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
+WFV_HD void riemann_hllc_n(const Phys &P, const double UL[4], const double UR[4], double nx, double ny, double F[4]) {
     // getHLLCRiemannFlux (include/Equations.hpp:381-483)
     Eos el, er;
     eos(P, UL, el);
     eos(P, UR, er);
-    const double nx = NX, ny = NY;
     const double rho_L = el.rho, rho_R = er.rho;
     const double u_L = UL[2] / rho_L, v_L = UL[3] / rho_L;
     const double u_R = UR[2] / rho_R, v_R = UR[3] / rho_R;
@@ -210,6 +211,15 @@ WFV_DEV void riemann_hllc(const Phys &P, const double UL[4], const double UR[4],
     const double Us[4] = {U_K[0], U_K[1], rho_K * (u_K + (S_star - vn_K) * nx), rho_K * (v_K + (S_star - vn_K) * ny)};
 #pragma unroll
     for (int r = 0; r < 4; ++r) F[r] = outside ? F_K[r] : S_K * (k * Us[r] - U_K[r]) + F_K[r];
+}
+
+template <int NX, int NY>
+WFV_HD void riemann_llf(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
+    riemann_llf_n(P, UL, UR, NX, NY, F);
+}
+template <int NX, int NY>
+WFV_HD void riemann_hllc(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
+    riemann_hllc_n(P, UL, UR, NX, NY, F);
 }
 
 // Mixed-policy node solvers on a face whose normal is x (NX = 1) or y.  Each side arrives as
@@ -288,7 +298,7 @@ WFV_DEV void llf_f(const Phys &P, const SideF &L, const SideF &R, const float ju
 
 // getRiemannFlux dispatch (include/Equations.hpp:329-360), resolved at compile time
 template <int FLUX, int NX, int NY>
-WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
+WFV_HD void riemann(const Phys &P, const double UL[4], const double UR[4], double F[4]) {
     if constexpr (FLUX == 1) riemann_hllc<NX, NY>(P, UL, UR, F);
     else riemann_llf<NX, NY>(P, UL, UR, F);
 }
@@ -297,7 +307,7 @@ WFV_DEV void riemann(const Phys &P, const double UL[4], const double UR[4], doub
 // getLEigenMatrix / getREigenMatrix (include/Equations.hpp:189-327), row-major 4x4,
 // built in double (they involve the equation of state) and used in W.
 template <class T, bool XDIR, bool FAST = false>
-WFV_DEV void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
+WFV_HD void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
     if constexpr (FAST) {
         // Mixed policy: the same matrices with the volume fraction from the division-free
         // equation of state (z1 = N/(D+N), z2 = D/(D+N)) and every quotient as a product
@@ -397,7 +407,7 @@ WFV_DEV void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
 
 // Array2D * Array1D (third_party/scTools/Array.h:604-626): C[i] = ((0 + A_i0 B0) + ...) + A_i3 B3
 template <class T>
-WFV_DEV void matvec(const T A[16], const T B[4], T C[4]) {
+WFV_HD void matvec(const T A[16], const T B[4], T C[4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         T acc = T(0);
@@ -407,12 +417,52 @@ WFV_DEV void matvec(const T A[16], const T B[4], T C[4]) {
     }
 }
 
+// ------------------------------------------------------------------ limiter
+// getPositivityPreservingTheta (src/WENOFV.cpp:1936-2000) of one cell in double, with the
+// operations of the stage kernel's all-FP64 limiter: U the cell average, F[side][v][g] its
+// face values (sides left, right, bottom, top), dX and dY the grid's global widths.
+WFV_HD double bp_theta(const Phys &P, const double U[4], const double F[4][4][4], double dX, double dY) {
+    constexpr double omega1 = 0.1;
+    constexpr double omegaR = 1.0 - 2 * omega1;
+    Eos e;
+    eos(P, U, e);
+    const double u = U[2] / (U[0] + U[1]);
+    const double v = U[3] / (U[0] + U[1]);
+    const double a1 = fabs(u * 1.0 + v * 0.0) + e.c;
+    const double a2 = fabs(u * 0.0 + v * 1.0) + e.c;
+    const double mu1 = (a1 / dX) / (a1 / dX + a2 / dY);
+    const double mu2 = (a2 / dY) / (a1 / dX + a2 / dY);
+    double th[2];
+    for (int k = 0; k < 2; ++k) {
+        const double uave = U[k];
+        const double eps = stdMin(1e-12, uave);
+        double rem = uave;
+        for (int g = 0; g < 4; ++g) {
+            const double cg = omega1 * GLW[g] / 2;
+            rem -= cg * mu2 * F[2][k][g];
+            rem -= cg * mu2 * F[3][k][g];
+            rem -= cg * mu1 * F[0][k][g];
+            rem -= cg * mu1 * F[1][k][g];
+        }
+        rem /= omegaR;
+        double m = rem;
+        for (int g = 0; g < 4; ++g) {
+            m = scMin(m, F[2][k][g]);
+            m = scMin(m, F[3][k][g]);
+            m = scMin(m, F[0][k][g]);
+            m = scMin(m, F[1][k][g]);
+        }
+        th[k] = scMin(fabs((uave - eps) / (uave - m)), 1.0);
+    }
+    return scMin(th[0], th[1]);
+}
+
 // ------------------------------------------------------------------ WENO
 // One 5-point stencil {a,b,c,d,e} = {u_mm, u_m, u, u_p, u_pp}; the smoothness indicators
 // are evaluated once and shared by every Gauss-Lobatto node reconstructed from the
 // stencil (the reference recomputes them per node with identical operands).
 template <class T>
-WFV_DEV void beta3(T a, T b, T c, T d, T e, T &b0, T &b1, T &b2) {
+WFV_HD void beta3(T a, T b, T c, T d, T e, T &b0, T &b1, T &b2) {
     // include/Utility.hpp:48-50 (and 270-272 for WENO-Z on the normalised stencil)
     T t;
     t = a - T(4) * b + T(3) * c; b0 = T(0.25) * (t * t);
@@ -431,7 +481,7 @@ struct Weno5 {
     T s0, s1, s2, s3, s4;
     T k0, k1, k2;  // JS: (eps+beta_k)^2; Z: tau/(beta_k+eps)
 
-    WFV_DEV Weno5(T a, T b, T c, T d, T e) : s0(a), s1(b), s2(c), s3(d), s4(e) {
+    WFV_HD Weno5(T a, T b, T c, T d, T e) : s0(a), s1(b), s2(c), s3(d), s4(e) {
         T b0, b1, b2;
         if constexpr (SCHEME == 0) {
             // WENO5threconstruction (include/Utility.hpp:48-61), eps = 1e-6
@@ -458,7 +508,7 @@ struct Weno5 {
     }
 
     template <int GP>
-    WFV_DEV T node() const {
+    WFV_HD T node() const {
         const T h0 = T(WENO_H[GP][0][0]) * s0 + T(WENO_H[GP][0][1]) * s1 + T(WENO_H[GP][0][2]) * s2;
         const T h1 = T(WENO_H[GP][1][0]) * s1 + T(WENO_H[GP][1][1]) * s2 + T(WENO_H[GP][1][2]) * s3;
         const T h2 = T(WENO_H[GP][2][0]) * s2 + T(WENO_H[GP][2][1]) * s3 + T(WENO_H[GP][2][2]) * s4;
@@ -552,7 +602,7 @@ using WenoSel = typename std::conditional<EXACT, Weno5<T, SCHEME>, WenoLin<T, SC
 // (include/Utility.hpp:473-663), in double.  s = {u_mmm, u_mm, u_m, u, u_p, u_pp, u_ppp}.
 
 // x^3 rounded once (the reference's pow(x, 3)): x^2 = p + e exactly, p x = hi + lo exactly
-WFV_DEV double cube_rn(double x) {
+WFV_HD double cube_rn(double x) {
     const double p = x * x;
     const double e = fma(x, x, -p);
     const double hi = p * x;
@@ -566,7 +616,7 @@ WFV_DEV double cube_rn(double x) {
 
 // fitted linear weights, cubic in l2h2 (include/Utility.hpp:347-380); pow(l, 2) is l*l
 template <int GP, bool FAST = false>
-WFV_DEV void wexp_d(double l, double &d0, double &d1, double &d2) {
+WFV_HD void wexp_d(double l, double &d0, double &d1, double &d2) {
     const double l2 = l * l, l3 = FAST ? l2 * l : cube_rn(l);
     if constexpr (GP == 0) {
         d0 = 0.3 - 0.057142858 * l + 0.0071428571 * l2 - 0.00077200577 * l3;
@@ -590,19 +640,21 @@ WFV_DEV void wexp_d(double l, double &d0, double &d1, double &d2) {
 // l2h2 = dif5/dif3 of the 6 cells biased toward the node (gp < 2: u_mmm..u_pp; else
 // u_mm..u_ppp), 0 where |dif3| < dx^4 (include/Utility.hpp:297-321)
 template <bool FAST = false>
-WFV_DEV double wexp_l2h2(const double *t, double dx4, bool clamp) {
+WFV_HD double wexp_l2h2(const double *t, double dx4, bool clamp) {
     const double dif5 = (-t[0] + 5.0 * t[1] - 10.0 * t[2] + 10.0 * t[3] - 5.0 * t[4] + t[5]);
     const double d6 = t[0] - 11.0 * t[1] + 28.0 * t[2] - 28.0 * t[3] + 11.0 * t[4] - t[5];
     const double dif3 = FAST ? d6 * (1.0 / 6.0) : d6 / 6.0;
     if (fabs(dif3) < dx4) return 0.0;
-    double l = FAST ? dif5 * rcp_nr(dif3) : dif5 / dif3;
+    double l;
+    if constexpr (FAST) l = dif5 * rcp_nr(dif3);
+    else l = dif5 / dif3;
     if (clamp && fabs(l) > 1) l = 1;
     return l;
 }
 
 // the inner five cells scaled to [0, 1] by their min and max (include/Utility.hpp:393-401)
 template <bool FAST = false>
-WFV_DEV void wexp_norm(const double s[7], double n[5]) {
+WFV_HD void wexp_norm(const double s[7], double n[5]) {
     if constexpr (FAST) {
         const double mx = fmax(fmax(fmax(fmax(s[1], s[2]), s[3]), s[4]), s[5]);
         const double mn = fmin(fmin(fmin(fmin(s[1], s[2]), s[3]), s[4]), s[5]);
@@ -617,7 +669,7 @@ WFV_DEV void wexp_norm(const double s[7], double n[5]) {
 
 // weights -> value: w_k = d_k (1 + a_k), normalised, sum w_k h_k (candidates as WENO5)
 template <int GP, bool FAST = false>
-WFV_DEV double wexp_combine(const double s[7], double d0, double d1, double d2, double a0, double a1, double a2) {
+WFV_HD double wexp_combine(const double s[7], double d0, double d1, double d2, double a0, double a1, double a2) {
     const double h0 = WENO_H[GP][0][0] * s[1] + WENO_H[GP][0][1] * s[2] + WENO_H[GP][0][2] * s[3];
     const double h1 = WENO_H[GP][1][0] * s[2] + WENO_H[GP][1][1] * s[3] + WENO_H[GP][1][2] * s[4];
     const double h2 = WENO_H[GP][2][0] * s[3] + WENO_H[GP][2][1] * s[4] + WENO_H[GP][2][2] * s[5];
@@ -633,7 +685,7 @@ template <bool FAST = false>
 struct WenoExp7 {
     const double *s;
     double dx4, a0, a1, a2, l_lo, l_hi;
-    WFV_DEV WenoExp7(const double *st, double dx4_) : s(st), dx4(dx4_) {
+    WFV_HD WenoExp7(const double *st, double dx4_) : s(st), dx4(dx4_) {
         double n[5];
         wexp_norm<FAST>(s, n);
         const double theta = 1;
@@ -656,7 +708,7 @@ struct WenoExp7 {
         l_hi = wexp_l2h2<FAST>(s + 1, dx4, true);
     }
     template <int GP>
-    WFV_DEV double node() const {
+    WFV_HD double node() const {
         double d0, d1, d2;
         wexp_d<GP, FAST>(GP >= 2 ? l_hi : l_lo, d0, d1, d2);
         return wexp_combine<GP, FAST>(s, d0, d1, d2, a0, a1, a2);

@@ -18,6 +18,8 @@ struct wfv_ctx {
     double *reg[3] = {nullptr, nullptr, nullptr};  // RK registers; reg[0] is the state
     double *d_dxC = nullptr, *d_dxW = nullptr, *d_dyC = nullptr;
     double *d_faces = nullptr, *d_theta = nullptr;
+    double *d_faces_ring = nullptr, *d_theta_ring = nullptr;  // wfv_face_values, [ny+2][nx+2]
+    double *d_cell_dt = nullptr;                               // wfv_cell_dt, [ny][nx]
     double *d_ghost = nullptr;        // fixed ghost ring (boundary 3)
     double *d_theta_flag = nullptr;   // limiter flags of the last stage (when recorded)
     wfv::StepCtl *d_ctl = nullptr;

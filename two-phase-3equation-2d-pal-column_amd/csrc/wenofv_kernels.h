@@ -165,6 +165,7 @@ struct StageArgs {
     unsigned long long *dtmin;       // if set: CFL minimum of the output state (bit pattern)
     double *faces_out;               // parity hook: limited face values [cell][side][v][g]
     double *theta_out;               // parity hook: limiter theta per cell
+    int faces_ring;                  // hooks over [ny+2][nx+2]: the cells and their ghost ring
     double *theta_flag;              // if set: theta < 1-1e-8 per cell, overwritten every stage
 };
 
@@ -174,7 +175,8 @@ struct DtArgs {
     double dX, dY, cfl;
     int nx, ny;
     unsigned *flags;
-    unsigned long long *dtmin;
+    unsigned long long *dtmin;  // CFL minimum (bit pattern) or nullptr
+    double *cell_dt;            // if set: each cell's CFL time step, [ny][nx]
 };
 
 // device-resident state of the reference time loop for one solver instance
