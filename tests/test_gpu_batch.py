@@ -22,7 +22,7 @@ def make_instances(seed, n):
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("rk,recon", [(2, 0), (1, 1), (0, 3)])
 def test_batch_instances_equal_single_solvers(precision, rk, recon):
     wls = make_instances(11 + rk, 5)

@@ -9,7 +9,8 @@ solver in test_gpu_full_runs.py).
 - C3, the 4096 x 2048 column with a 16-mode radius perturbation, 200 steps: mixed against
   all-FP64 within 1e-5 relative L1 on z1, rho, u, v, with the same properties.
 - C4, the 256 seeded instances with grids {64..512}^2, 500 steps in one batch, mixed and
-  all-FP64: the batch within 1e-5 relative L1 on z1, rho, u, v, every instance with the
+  all-FP64: the batch within 1e-5 relative L1 on z1, rho, u, v (precision 2, whose
+  reconstruction is FP64 too, also on the conserved fields), every instance with the
   properties; and one all-FP64 batch of an instance of every grid size equal to the oracle
   bit for bit over 5 steps at full size.
 - C5, 8192 x 8192 with 64 columns and a ratio-3 step, 20 steps: mixed against all-FP64 within
@@ -131,23 +132,38 @@ def run_batch(wls, steps, precision, idx=None):
     return out
 
 
-def test_c4_batch256_mixed_against_fp64(c4):
-    steps = c4[0].steps
-    print(f"C4 batch256 mixed and FP64, {steps} steps", flush=True)
-    Sm, _ = run_batch(c4, steps, 1)
-    Sd, _ = run_batch(c4, steps, 0)
-    num = {k: 0.0 for k in ("z1", "rho", "u", "v")}
-    den = dict(num)
-    for w, Um, Ud in zip(c4, Sm, Sd):
-        check_properties(Um, w.U0, w.params, MASS_TOL[1])
+@pytest.fixture(scope="module")
+def c4_fp64(c4):
+    S, _ = run_batch(c4, c4[0].steps, 0)
+    for w, Ud in zip(c4, S):
         check_properties(Ud, w.U0, w.params, MASS_TOL[0])
+    return S
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_c4_batch256_mixed_against_fp64(c4, c4_fp64, precision):
+    """The batch within 1e-5 relative L1 of all-FP64 on z1, rho, u, v; precision 2 (FP64
+    reconstruction) also on the conserved fields (z1 rho1, z2 rho2, rho u, rho v)."""
+    steps = c4[0].steps
+    print(f"C4 batch256 precision {precision} and FP64, {steps} steps", flush=True)
+    Sm, _ = run_batch(c4, steps, precision)
+    keys = ("z1", "rho", "u", "v", "z1rho1", "z2rho2", "rhou", "rhov")
+    num = {k: 0.0 for k in keys}
+    den = dict(num)
+    for w, Um, Ud in zip(c4, Sm, c4_fp64):
+        check_properties(Um, w.U0, w.params, MASS_TOL[1])
         n_, d_ = gate_errors(Um, Ud, w.params)
-        for k in num:
+        mom = np.hypot(Ud[2], Ud[3]).sum()
+        for r, k in enumerate(("z1rho1", "z2rho2", "rhou", "rhov")):
+            n_[k] = np.abs(Um[r] - Ud[r]).sum()
+            d_[k] = mom if r >= 2 else np.abs(Ud[r]).sum()
+        for k in keys:
             num[k] += n_[k]
             den[k] += d_[k]
-    errs = {k: num[k] / den[k] for k in num}
-    print(f"C4 mixed vs FP64 rel L1 over the batch: {errs}", flush=True)
-    assert max(errs.values()) <= 1e-5, errs
+    errs = {k: num[k] / den[k] for k in keys}
+    print(f"C4 precision {precision} vs FP64 rel L1 over the batch: {errs}", flush=True)
+    gate = keys if precision == 2 else keys[:4]
+    assert max(errs[k] for k in gate) <= 1e-5, errs
 
 
 def test_c4_every_grid_size_matches_oracle_bitwise(c4):

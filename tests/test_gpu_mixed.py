@@ -43,17 +43,18 @@ def field_errors(U32, U64, params):
 
 @pytest.mark.parametrize("recon", [0, 1, 2, 3])
 @pytest.mark.parametrize("scheme", [0, 1])
-def test_mixed_matches_fp64_liquid_column(recon, scheme):
+@pytest.mark.parametrize("mixed", [1, 2])
+def test_mixed_matches_fp64_liquid_column(recon, scheme, mixed):
     wl = cases.liquid_column(nx=256, ny=128)
-    print(f"mixed vs fp64 recon{recon} scheme{scheme}", flush=True)
+    print(f"mixed (precision {mixed}) vs fp64 recon{recon} scheme{scheme}", flush=True)
     out = {}
-    for prec in (0, 1):
+    for prec in (0, mixed):
         sol = wenofv.Solver(wl.nx, wl.ny, **wl.solver_kwargs(precision=prec, recon=recon, scheme=scheme))
         sol.set_state(wl.U0)
         n, t, dts = sol.advance(fixed_steps=60, max_dts=64)
         out[prec] = (sol.get_state(), dts, sol.flags())
-    U64, U32 = out[0][0], out[1][0]
-    assert out[1][2] == 0 and np.isfinite(U32).all()
+    U64, U32 = out[0][0], out[mixed][0]
+    assert out[mixed][2] == 0 and np.isfinite(U32).all()
     err = field_errors(U32, U64, wl.params)
     print({k: f"{v:.2e}" for k, v in err.items()}, flush=True)
     tol = TOL

@@ -216,6 +216,10 @@ struct FaceSide<PrecMixed> {
     }
 };
 
+// precision 2 keeps the mixed policy's face values (only its reconstruction differs)
+template <>
+struct FaceSide<PrecMixedW64> : FaceSide<PrecMixed> {};
+
 // Riemann flux through one face, integrated over its 4 Gauss-Lobatto nodes
 // (src/WENOFV.cpp:1707-1746 fluxes, 1764-1799 quadrature).
 template <class PR, int FLUX, int NX, int NY>

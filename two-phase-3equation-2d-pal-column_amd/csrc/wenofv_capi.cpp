@@ -40,7 +40,8 @@ const char *check_options(const wfv_options *opt, const wfv_case *cs) {
     if (opt->scheme < 0 || opt->scheme > 2) return "SCHEME must be 0 (WENO-JS), 1 (WENO-Z) or 2 (WENO-EXP)";
     if (opt->riemann_flux < 0 || opt->riemann_flux > 1) return "RIEMANNFLUX must be 0 or 1";
     if (opt->reconstruction < 0 || opt->reconstruction > 4) return "RECONSTRUCTIONTYPE must be 0..4";
-    if (opt->precision < 0 || opt->precision > 1) return "precision must be 0 (fp64) or 1 (mixed)";
+    if (opt->precision < 0 || opt->precision > 2)
+        return "precision must be 0 (fp64), 1 (mixed) or 2 (mixed, reconstruction in fp64)";
     if (opt->boundary < 0 || opt->boundary > 3)
         return "boundary must be 0 (periodic), 1 (slip walls), 2 (no-slip walls) or 3 (fixed ghost cells)";
     if (!(opt->cfl > 0)) return "CFL must be positive";
@@ -169,7 +170,9 @@ int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active,
 
 extern "C" {
 
-const char *wfv_version(void) { return "wenofv-gfx950 1.1 (fp64, mixed; js, z, exp; cons, char, prim, prim2, prim3)"; }
+const char *wfv_version(void) {
+    return "wenofv-gfx950 1.2 (fp64, mixed, mixed-w64; js, z, exp; cons, char, prim, prim2, prim3)";
+}
 
 int wfv_create(const wfv_options *opt, const wfv_case *cs, wfv_ctx **out) {
     if (!opt || !cs || !out) return WFV_ERR_ARG;
@@ -623,7 +626,7 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     else if (opt->scheme < 0 || opt->scheme > 2) why = "SCHEME must be 0, 1 or 2";
     else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
     else if (opt->reconstruction < 0 || opt->reconstruction > 4) why = "RECONSTRUCTIONTYPE must be 0..4";
-    else if (opt->precision < 0 || opt->precision > 1) why = "precision must be 0 or 1";
+    else if (opt->precision < 0 || opt->precision > 2) why = "precision must be 0, 1 or 2";
     else if (!(opt->cfl > 0)) why = "CFL must be positive";
     for (int i = 0; i < n && why.empty(); ++i) {
         const wfv_instance &I = inst[i];

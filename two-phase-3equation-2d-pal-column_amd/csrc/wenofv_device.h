@@ -39,6 +39,14 @@ struct PrecMixed {
     using F = float;
     static constexpr bool exact = false;
 };
+// PrecMixedW64 (precision 2): the mixed policy with the 5-point reconstruction in double, as
+// the mixed characteristic and 7-point forms already reconstruct; the Riemann-flux arithmetic
+// stays FP32.  FP32 reconstruction is what sets the mixed policy's error against all-FP64.
+struct PrecMixedW64 {
+    using W = double;
+    using F = float;
+    static constexpr bool exact = false;
+};
 
 // per-case constants; the host fills them with the reference's own expressions
 // (c_k^2 = g_sound_k * g_sound_k, q = rho20*c2^2 - rho10*c1^2, include/Equations.hpp:86-92)
@@ -111,11 +119,9 @@ WFV_DEV void eos_fast(const Phys &P, const double U[4], Eos &e) {
     e.c = DN * rsq_nr(e.rho * (N * N * rcp_nr(a * P.c1_2) + 2.0 * D));
 }
 
-// The mixed Riemann solvers' equation of state: the pressure as eos_fast (the sound-speed
-// root's square root keeps its Newton step, the x = 0 guard folded into a max), the sound
-// speed, which enters only float wave speeds, from the bare hardware reciprocal and inverse
-// square root estimates (relative error <= 5.2e-8 over 1e-30..1e30 on gfx950, below float
-// rounding's 6e-8).
+// The mixed Riemann solvers' equation of state: the pressure as eos_fast (the x = 0 guard of
+// the discriminant's square root folded into a max) and the sound speed with Newton-refined
+// reciprocal and inverse square root, both to double accuracy.
 WFV_DEV void eos_flux(const Phys &P, double a, double b, Eos &e) {
     const double t1 = a * P.c1_2, t2 = b * P.c2_2;
     const double dq = P.q - (t2 - t1);
@@ -125,7 +131,7 @@ WFV_DEV void eos_flux(const Phys &P, double a, double b, Eos &e) {
     const double DN = D + N;
     e.rho = a + b;
     e.p = 0.5 * DN - P.c2_2 * P.rho20;
-    e.c = DN * __builtin_amdgcn_rsq(e.rho * (N * N * __builtin_amdgcn_rcp(t1) + 2.0 * D));
+    e.c = DN * rsq_nr(e.rho * (N * N * rcp_nr(t1) + 2.0 * D));
 }
 
 template <bool FAST>
