@@ -1,11 +1,14 @@
 """bench.py's contract pieces that need no GPU: one GPU only (a request for more exits with a
-message), and the committed rocprofv3 summaries carry every machine-readable figure the
-bench line reads from them (roofline traffic, VALU busy and issue fractions, instructions
-per cell-stage, the profiled stage duration)."""
+message); the committed rocprofv3 summaries carry every machine-readable figure the bench
+line reads from them (roofline traffic, VALU busy and issue fractions, instructions per
+cell-stage, the profiled stage duration); the roofline block is the VALU bound with the HBM
+one beside it; the accuracy block counts instances against the tolerance and the FP64
+spread and locates the ones beyond both."""
 import os
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 from conftest import REPO
@@ -34,3 +37,37 @@ def test_profile_summaries_hold_the_bench_figures(name):
     cells = 14315520
     assert abs(bench.profiled(name, "algorithmic_bytes_per_launch") - bench.BYTES_PER_CELL_STAGE * cells) < 2
     assert bench.profiled(name, "traffic_bytes_per_launch") >= bench.profiled(name, "algorithmic_bytes_per_launch")
+
+
+def test_roofline_is_the_valu_bound_with_the_hbm_one_beside_it():
+    opts = dict(scheme=1, recon=0, flux=1, bp=1)
+    cells, stage_ms = 1_000_000, 0.1
+    r = bench.roofline(cells, stage_ms, opts, traffic=1.0e8)
+    assert r["bound"] == "valu" and r["unit"] == "TFLOP/s" and r["peak"] == 78.6
+    assert r["achieved"] == pytest.approx(3868.3 * cells / (stage_ms * 1e-3) / 1e12)
+    assert r["frac"] == pytest.approx(r["achieved"] / 78.6)
+    h = r["hbm"]
+    assert h["bound"] == "hbm" and h["unit"] == "GB/s" and h["traffic"] == 1.0e8
+    assert h["achieved"] == pytest.approx(256.0 / 3.0 * cells / (stage_ms * 1e-3) / 1e9)
+    # an option set without an operation count falls back to the HBM roofline
+    assert bench.roofline(cells, stage_ms, dict(opts, scheme=0))["bound"] == "hbm"
+
+
+def test_accuracy_block_counts_instances_against_tolerance_and_spread():
+    import cases
+    wls = [cases.batch_instance(sp) for sp in cases.batch_specs(3, seed=4)]
+    Sd = [w.U0.copy() for w in wls]
+    rng = np.random.default_rng(1)
+    # instance 0 exact, 1 off by 1e-9 relative, 2 off by 1e-3 in the momenta at one cell
+    S = [Sd[0].copy(), Sd[1] * (1 + 1e-9 * rng.standard_normal(Sd[1].shape)), Sd[2].copy()]
+    w2 = wls[2]
+    j, i = w2.ny // 2, w2.nx // 2
+    S[2][2:, j, i] += 1e-3 * np.abs(Sd[2][:2, j, i].sum()) + 1.0
+    spread = [s * (1 + 1e-12) for s in Sd]
+    acc = bench.accuracy_block(wls, S, Sd, spread, spread, 500)
+    pi = acc["per_instance_max_field_rel_l1"]
+    assert acc["steps"] == 500 and pi["fraction_within_tolerance"] == pytest.approx(2 / 3)
+    assert acc["fraction_within_tolerance_or_fp64_6e-8_spread"] == pytest.approx(2 / 3)
+    (b,) = acc["beyond_tolerance_and_spread"]
+    assert b["instance"] == 2 and b["field"] in ("u", "v") and (b["where"]["i"], b["where"]["j"]) == (i, j)
+    assert acc["rel_l1_per_field_batch"]["z1rho1"] < 1e-8

@@ -1,6 +1,7 @@
-// Generated by tests/test_weno_constants.py's formulas (the reference expressions of
-// include/Utility.hpp:12-39 and 64-90 evaluated in IEEE double); hexfloat so that the
-// device compiler needs no library call.  tests/test_weno_constants.py re-derives them.
+// The reference expressions of include/Utility.hpp:12-39 and 64-90 evaluated in IEEE
+// double, written as hexfloats so that the device compiler needs no library call.
+// tests/test_weno_constants.py evaluates those expressions again and compares bit for bit,
+// and checks the tables against polynomial exactness at the Gauss-Lobatto nodes.
 #pragma once
 namespace wfv {
 // linear weights WENO_D[gp][k]
