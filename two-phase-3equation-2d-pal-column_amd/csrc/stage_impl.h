@@ -39,6 +39,15 @@ namespace wfv {
 
 constexpr int LANES = WAVE_LANES;
 
+// Row prefetch (stage_body's PF) in a kernel: where one wavefront per SIMD leaves nothing
+// to hide a load's latency, and in the single-domain mixed 5-point conservative and
+// characteristic kernels, whose small grids run a few wavefronts in all (they have the
+// registers for it)
+template <class PR, int RECON, int SCHEME, int MINW>
+constexpr bool prefetch_rows() {
+    return MINW == 1 || (!PR::exact && SCHEME != 2 && RECON <= 1);
+}
+
 WFV_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -135,13 +144,25 @@ WFV_DEV void from_prim(const Phys &P, const W w[4], F out[4]) {
     out[0] = F(z1rho1); out[1] = F(z2rho2); out[2] = F(rho * double(w[2])); out[3] = F(rho * double(w[3]));
 }
 
-template <bool EX, class W, int SCHEME>
-WFV_DEV void recon4(const W s[5], W out[4]) {
-    WenoSel<EX, W, SCHEME> st(s[0], s[1], s[2], s[3], s[4]);
-    out[0] = st.template node<0>();
-    out[1] = st.template node<1>();
-    out[2] = st.template node<2>();
-    out[3] = st.template node<3>();
+// the 4 Gauss-Lobatto node values of a 5-point stencil; nodes outside `nodes` (bit g,
+// wave-uniform) are left at 0
+template <bool EX, class W, int SCHEME, bool ZC = false>
+WFV_DEV void recon4(const W s[5], W out[4], unsigned nodes = 0xFu) {
+    WenoSel<EX, W, SCHEME, ZC> st(s[0], s[1], s[2], s[3], s[4]);
+    out[0] = out[1] = out[2] = out[3] = W(0);
+    if (nodes & 1u) out[0] = st.template node<0>();
+    if (nodes & 2u) out[1] = st.template node<1>();
+    if (nodes & 4u) out[2] = st.template node<2>();
+    if (nodes & 8u) out[3] = st.template node<3>();
+}
+
+// a wave-uniform union of per-lane bit sets (the low 4 bits)
+WFV_DEV unsigned wave_or4(unsigned bits) {
+    unsigned r = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if (__ballot((bits >> b) & 1u) != 0) r |= 1u << b;
+    return r;
 }
 
 // Register rotation for loops kept rolled to bound the code size: slot 0 always holds the
@@ -259,20 +280,26 @@ WFV_DEV void face_flux(const Phys &P, FaceSide<PR> &a, const double ba[4], FaceS
         for (int g = 0; g < 4; ++g) {
             const SideF L = {a.r[0][g], a.r[1][g], ban + a.m[N - 2][g], bat + a.m[T - 2][g]};
             const SideF R = {b.r[0][g], b.r[1][g], bbn + b.m[N - 2][g], bbt + b.m[T - 2][g]};
-            float fl[4];
             double pref;
+            const double qw = GLW[g] * width / 2.0;
             if constexpr (FLUX == 1) {
-                hllc_f(P, L, R, fl, pref);
+                double fd[2];
+                float fm[2];
+                hllc_f(P, L, R, fd, fm, pref);
+                G[0] += fd[0] * qw;
+                G[1] += fd[1] * qw;
+                G[N] += (double(fm[0]) + pref) * qw;
+                G[T] += double(fm[1]) * qw;
             } else {
+                float fl[4];
                 const float jump[4] = {float(R.a - L.a), float(R.b - L.b), djn + (b.m[N - 2][g] - a.m[N - 2][g]),
                                        djt + (b.m[T - 2][g] - a.m[T - 2][g])};
                 llf_f(P, L, R, jump, fl, pref);
+                G[0] += double(fl[0]) * qw;
+                G[1] += double(fl[1]) * qw;
+                G[N] += (double(fl[2]) + pref) * qw;
+                G[T] += double(fl[3]) * qw;
             }
-            const double qw = GLW[g] * width / 2.0;
-            G[0] += double(fl[0]) * qw;
-            G[1] += double(fl[1]) * qw;
-            G[N] += (double(fl[2]) + pref) * qw;
-            G[T] += double(fl[3]) * qw;
         }
         if (check) {
 #pragma unroll
@@ -341,6 +368,11 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr int R = ST::R, NW = ST::NW, HALO = ST::HALO, SW = ST::SW;
     constexpr bool EX = PR::exact;
     constexpr bool DEV = !EX && R == 2;
+    constexpr bool TRIM = R == 2 && RECON <= 1;
+    // EARLY (mixed policy): a row's height and its RK base state are loaded well before their
+    // use, so that their latency overlaps arithmetic; the all-FP64 kernels, whose registers
+    // are spent, load them where they are used
+    constexpr bool EARLY = !EX;
     // Loops over the 4 variables or characteristic fields: unrolled (no register rotation,
     // fewer spills; measured faster in every kernel, the all-FP64 ones included, although
     // several of those then outgrow the 64 KiB instruction cache).
@@ -349,7 +381,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 
     const StripPlan &sp = A.sp;
     if (strip >= sp.total) return;
-    if (A.active && *A.active == 0) return;  // device-side time loop already finished
+    // the device-side time loop's flag; the wavefront returns if the loop has finished, once
+    // the loads that every strip starts with are in flight (one memory latency, not three)
+    const int act = A.active ? *A.active : 1;
     const int nx = A.nx, ny = A.ny;
     const size_t nc = (size_t)nx * ny;
     // this lane's segment (segment_of); stencils and face exchanges stay inside it
@@ -370,28 +404,28 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     const double cdt = A.c * dt;
     const double dxc = owned ? A.dxC[cx] : 1.0;
     const double dxw = owned ? A.dxW[cx] : 1.0;
-    // mixed policy: reciprocals of the widths replace the reference's divisions by them
-    const double idxc = EX ? 0.0 : 1.0 / dxc;
-    const double idxw = EX ? 0.0 : 1.0 / dxw;
-    const double idX = EX ? 0.0 : 1.0 / A.dX;
-    const double idY = EX ? 0.0 : 1.0 / A.dY;
     constexpr double omega1 = 0.1;
     constexpr double omegaR = 1.0 - 2 * omega1;
 
     // register window: cell averages of rows r-R..r+R of this lane's column, and their
-    // reconstructed (primitive) variables for RECON 2, 3, 4
+    // reconstructed (primitive) variables for RECON 2, 3, 4.  A row's cell is a fixed ghost
+    // cell (boundary 3) from the ring or a cell of the state through the periodic or wall
+    // map: one load per variable either way, with no branch, so that the window's loads are
+    // all in flight together.
     double win[NW][4];
     double pwin[NW][4];
     auto load_row = [&](int row, double out[4]) {
-        if (A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny)) {
-            const size_t rs = ring_size(nx, ny), o = ghost_offset(nx, ny, xc, row);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) out[v] = ((const WFV_GLOBAL double *)A.ghost)[v * rs + o];
-            return;
-        }
         bool mv;
         const int cy = map_index(row, ny, A.bc, mv);
-        load_cell<double>(A.U, nc, (size_t)cy * nx + cx, mu, mv, A.bc, out);
+        const bool g = A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny);
+        const WFV_GLOBAL double *src = (const WFV_GLOBAL double *)(g ? A.ghost : A.U);
+        const size_t stride = g ? ring_size(nx, ny) : nc;
+        const size_t off = g ? ghost_offset(nx, ny, xc, row) : (size_t)cy * nx + cx;
+        const double a = src[off], b = src[stride + off], c = src[2 * stride + off], d = src[3 * stride + off];
+        // slip walls negate the momentum normal to each mirror, no-slip walls both components
+        const bool nu = !g && (A.bc == 2 ? mu != mv : mu);
+        const bool nv = !g && (A.bc == 2 ? mu != mv : mv);
+        out[0] = a; out[1] = b; out[2] = nu ? -c : c; out[3] = nv ? -d : d;
     };
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
@@ -416,10 +450,30 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     if constexpr (PF) {
         if (nrow_w >= 0) load_row(y0 + R, ahead);
     }
+    // mixed policy: reciprocals of the widths replace the reference's divisions by them
+    const double idxc = EX ? 0.0 : 1.0 / dxc;
+    const double idxw = EX ? 0.0 : 1.0 / dxw;
+    const double idX = EX ? 0.0 : 1.0 / A.dX;
+    const double idY = EX ? 0.0 : 1.0 / A.dY;
+    asm volatile("" ::: "memory");  // the loads above are issued before the flag is tested
+    if (act == 0) return;
+    // height of the previous row and (mixed) its reciprocal, for the RK completion of that row
+    double dy_prev = 1.0, idy_prev = 0.0;
 
     for (int it = 0; it <= nrow_w + 1; ++it) {
         const int r = y0 - 1 + it;
         const bool live = it <= nrow + 1;  // this lane's segment still has rows
+        // Node rows (bit gy) the sweeps must reconstruct: all four in an owned row; in the rows
+        // below and above the segment, whose cells only lend it their north (gy = 3) or south
+        // (gy = 0) face, just that row, unless the limiter needs the cells' other faces too or
+        // every face value is recorded.  A union over the wavefront's lanes.
+        // (The primitive and 7-point forms reconstruct every row: their register budget is spent.)
+        const bool full_l = bp || A.faces_out || (it >= 1 && it <= nrow);
+        // this row's height, loaded here so that its latency overlaps the sweeps
+        const double dy_row = (EARLY && it >= 1 && it <= nrow) ? ((const WFV_GLOBAL double *)A.dyC)[r] : 1.0;
+        const unsigned rows =
+            !TRIM ? 0xFu
+                  : wave_or4(!live ? 0u : (full_l ? 0xFu : ((it == 0 ? 0x8u : 0u) | (it == nrow + 1 ? 0x1u : 0u))));
         if (it > 0) {
 #pragma unroll
             for (int k = 0; k < NW - 1; ++k)
@@ -495,7 +549,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int k = 0; k < 4; ++k) {  // characteristic field k sits in slot 0
                     W s5[5] = {pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]};
                     W o[4];
-                    recon4<EX, W, SCHEME>(s5, o);
+                    recon4<EX, W, SCHEME>(s5, o, rows);
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
                         rotate_push(yc[g], o[g]);
@@ -520,7 +574,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int v = 0; v < 4; ++v) {  // variable v sits in slot 0
                     W s5[5] = {sw[0][0], sw[1][0], sw[2][0], sw[3][0], sw[4][0]};
                     W o[4];
-                    recon4<EX, W, SCHEME>(s5, o);
+                    recon4<EX, W, SCHEME, DEV>(s5, o, rows);
 #pragma unroll
                     for (int g = 0; g < 4; ++g) L.y[g * 4 + v][lane] = o[g];
 #pragma unroll
@@ -614,6 +668,13 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 eigen_LR<W, true, !EX>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int gy = 0; gy < 4; ++gy) {
+                    if (!(rows & (1u << gy))) {
+#pragma unroll
+                        for (int gx = 0; gx < 4; ++gx)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) ug[gx][gy][v] = W(0);
+                        continue;
+                    }
                     W pc[5][4];
 #pragma unroll
                     for (int t = 0; t < 5; ++t) {
@@ -646,6 +707,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 for (int gy = 0; gy < 4; ++gy)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
+#pragma unroll
+                        for (int gx = 0; gx < 4; ++gx) ug[gx][gy][v] = W(0);
+                        if (!(rows & (1u << gy))) continue;
                         WenoSel<EX, W, SCHEME> st(xv(gy, v, 0), xv(gy, v, 1), xv(gy, v, 2), xv(gy, v, 3), xv(gy, v, 4));
                         ug[0][gy][v] = st.template node<0>();
                         ug[3][gy][v] = st.template node<3>();
@@ -658,21 +722,34 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 double bc[4] = {Bc[0], Bc[1], Bc[2], Bc[3]};
 #pragma unroll VU
                 for (int v = 0; v < 4; ++v) {  // variable v's base sits in bc[0]
+                    // the neighbours' bases relative to this cell's, shared by the node rows
+                    W bd[5];
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) bd[t] = t == 2 ? W(0) : W(L.base[v][lane - 2 + t] - bc[0]);
 #pragma unroll
                     for (int gy = 0; gy < 4; ++gy) {
-                        W st5[5];
+                        W n[4] = {W(0), W(0), W(0), W(0)};
+                        if (rows & (1u << gy)) {
+                            W st5[5];
 #pragma unroll
-                        for (int t = 0; t < 5; ++t) {
-                            const int l = lane - 2 + t;
-                            // the centre's base is bc[0] itself: its value is its y-deviation
-                            st5[t] = t == 2 ? L.y[gy * 4 + v][l] : W(L.base[v][l] - bc[0]) + L.y[gy * 4 + v][l];
+                            for (int t = 0; t < 5; ++t) {
+                                // the centre's base is bc[0] itself: its value is its y-deviation
+                                const W y = L.y[gy * 4 + v][lane - 2 + t];
+                                st5[t] = t == 2 ? y : bd[t] + y;
+                            }
+                            WenoSel<EX, W, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
+                            n[0] = st.template node<0>();
+                            n[3] = st.template node<3>();
+                            if (gy == 0 || gy == 3) {
+                                n[1] = st.template node<1>();
+                                n[2] = st.template node<2>();
+                            }
                         }
-                        WenoSel<EX, W, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
-                        rotate_push(ug[0][gy], st.template node<0>());
-                        rotate_push(ug[3][gy], st.template node<3>());
+                        rotate_push(ug[0][gy], n[0]);
+                        rotate_push(ug[3][gy], n[3]);
                         if (gy == 0 || gy == 3) {
-                            rotate_push(ug[1][gy], st.template node<1>());
-                            rotate_push(ug[2][gy], st.template node<2>());
+                            rotate_push(ug[1][gy], n[1]);
+                            rotate_push(ug[2][gy], n[2]);
                         }
                     }
                     rotate_left(bc);
@@ -809,6 +886,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 
         if (it > 0) {
             const bool interior_row = it <= nrow;
+            // the RK base state of row r-1, loaded ahead of the fluxes that precede its use
+            double Ub[4];
+            const bool complete = it >= 2 && owned && live;
+            if (EARLY && complete && A.Un) load_cell<double>(A.Un, nc, (size_t)(r - 1) * nx + xc, false, false, 0, Ub);
             // ---- 3. face below row r: Riemann problem (top of r-1 | bottom of r)
             double G[4] = {0.0, 0.0, 0.0, 0.0};
             if (owned && live) {
@@ -829,7 +910,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                     for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
                 wave_sync();
-                dyr = ((const WFV_GLOBAL double *)A.dyC)[r];
+                dyr = EARLY ? dy_row : ((const WFV_GLOBAL double *)A.dyC)[r];
                 if constexpr (!EX) idyr = rcp_nr(dyr);
                 if (xface_lane) {
                     // the neighbour's west face values, as exact FP64 states (base 0)
@@ -864,17 +945,16 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             }
             // ---- 5. complete row r-1: -north face, source, RK combination (src/WENOFV.cpp:1816-1827,
             //      1831-1872, 466-471)
-            if (it >= 2 && owned && live) {
+            if (complete) {
                 const int rr = r - 1;
-                const double dyp = ((const WFV_GLOBAL double *)A.dyC)[rr];
-                const double idyp = EX ? 0.0 : rcp_nr(dyp);
+                const double dyp = EARLY ? dy_prev : ((const WFV_GLOBAL double *)A.dyC)[rr];
+                const double idyp = idy_prev;
                 const double *Ucur = win[R - 1];
-                double Ub[4];
-                if (A.Un) {
-                    load_cell<double>(A.Un, nc, (size_t)rr * nx + xc, false, false, 0, Ub);
-                } else {
+                if (!A.Un) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) Ub[v] = Ucur[v];
+                } else if (!EARLY) {
+                    load_cell<double>(A.Un, nc, (size_t)rr * nx + xc, false, false, 0, Ub);
                 }
                 double Unew[4];
 #pragma unroll
@@ -912,6 +992,8 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     else rhs_part[v] = rhs_x[v] + G[v] * (idxc * idyr);
                 }
             }
+            dy_prev = dyr;
+            idy_prev = idyr;
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -937,7 +1019,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
-    stage_body<PR, RECON, SCHEME, FLUX, MINW == 1>(A, blockIdx.x, L);
+    stage_body<PR, RECON, SCHEME, FLUX, prefetch_rows<PR, RECON, SCHEME, MINW>()>(A, blockIdx.x, L);
 }
 
 // Batch of independent instances with uneven grids in one persistent launch: the

@@ -119,15 +119,18 @@ WFV_DEV void eos_fast(const Phys &P, const double U[4], Eos &e) {
     e.c = DN * rsq_nr(e.rho * (N * N * rcp_nr(a * P.c1_2) + 2.0 * D));
 }
 
-// The mixed Riemann solvers' equation of state: the pressure as eos_fast (the x = 0 guard of
-// the discriminant's square root folded into a max) and the sound speed with Newton-refined
-// reciprocal and inverse square root, both to double accuracy.
+// The mixed Riemann solvers' equation of state: the pressure as eos_fast and the sound speed
+// with Newton-refined reciprocal and inverse square root, both to double accuracy.  The
+// discriminant's square root S = x y (1.5 - 0.5 x y^2) = g (1.5 - 0.5 g y) with y the inverse
+// square root estimate and g = x y: one Newton step, applied to g; at x = 0 (y taken at a
+// tiny floor) g and S are 0.
 WFV_DEV void eos_flux(const Phys &P, double a, double b, Eos &e) {
     const double t1 = a * P.c1_2, t2 = b * P.c2_2;
     const double dq = P.q - (t2 - t1);
     const double x = dq * dq + 4.0 * t1 * t2;
-    const double S = x * rsq_nr(fmax(x, 1.0e-300));  // 0 at x = 0
-    const double N = dq + S, D = 2.0 * t2;
+    const double y = __builtin_amdgcn_rsq(fmax(x, 1.0e-300));
+    const double g = x * y;
+    const double N = g * ((g * y) * -0.5 + 1.5) + dq, D = 2.0 * t2;
     const double DN = D + N;
     e.rho = a + b;
     e.p = 0.5 * DN - P.c2_2 * P.rho20;
@@ -241,7 +244,7 @@ struct SideF {
     float mn, mt; // normal and tangential momentum
 };
 
-WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, float F[4], double &pref) {
+WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, double Fd[2], float F[2], double &pref) {
     Eos el, er;
     eos_flux(P, L.a, L.b, el);
     eos_flux(P, R.a, R.b, er);
@@ -250,7 +253,6 @@ WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, float F[4], d
     const float irho_L = __builtin_amdgcn_rcpf(rho_L);
     const float vn_L = L.mn * irho_L, vn_R = R.mn * __builtin_amdgcn_rcpf(rho_R);
     const float pre_R = float(er.p - el.p), c_L = float(el.c), c_R = float(er.c);
-    const float aL = float(L.a), bL = float(L.b), aR = float(R.a), bR = float(R.b);
     // Roe weights sqrt(rho_K) / (sqrt(rho_L) + sqrt(rho_R)) from one square root:
     // w_L = 1 / (1 + t), w_R = t w_L with t = sqrt(rho_R / rho_L)
     const float t = __builtin_amdgcn_sqrtf(rho_R * irho_L);
@@ -264,23 +266,24 @@ WFV_DEV void hllc_f(const Phys &P, const SideF &L, const SideF &R, float F[4], d
     const float S_R = __builtin_fmaxf(vn_R + c_R, vn_roe + c_roe);
     const float qL = (S_L - vn_L) * rho_L, qR = (S_R - vn_R) * rho_R;
     const float S_star = (qL * vn_L - qR * vn_R + pre_R) * __builtin_amdgcn_rcpf(qL - qR);
+    // upwind side K; the flux F_K + S'(U*_K - U_K) with S' = min(S_L, 0) (left) or
+    // max(S_R, 0) (right), which is F_K outside the fan.  With U*_K = k (a, b, rho S*, m_t)_K,
+    // k = (S_K - vn_K)/(S_K - S*) = 1 + e, e = (S* - vn_K)/(S_K - S*), the transported
+    // components move with phi = vn_K + S' e and the normal momentum flux is
+    // m_K vn_K + p_K + S' e S_K rho_K.  Where S' = 0 (outside the fan) the correction is 0
+    // exactly, whatever e is (S_K = S* would make it infinite).
     const bool left = S_star >= 0.0f;
-    const bool outside = left ? (S_L >= 0.0f) : (S_R <= 0.0f);
     const float S_K = left ? S_L : S_R, vn_K = left ? vn_L : vn_R, rho_K = left ? rho_L : rho_R;
-    const float q_K = left ? qL : qR, p_K = left ? 0.0f : pre_R;
-    const float U_K[4] = {left ? aL : aR, left ? bL : bR, left ? L.mn : R.mn, left ? L.mt : R.mt};
-    // F_K + S_K (U*_K - U_K) with U*_K = k (a, b, rho S*, m_t)_K, k = (S_K - vn_K)/(S_K - S*),
-    // in closed form: k (S_K - S*) = S_K - vn_K turns the transported components into
-    // U_K k S* and the normal momentum into (k rho_K S*) S* + p*, where
-    // p* = p_K + q_K (S* - vn_K); outside the fan F_K = U_K vn_K (+ p_K)
-    const float k = (S_K - vn_K) * __builtin_amdgcn_rcpf(S_K - S_star);
-    const float phi = outside ? vn_K : k * S_star;
-    const float M = outside ? U_K[2] : rho_K * S_star;
-    const float Pn = outside ? p_K : q_K * (S_star - vn_K) + p_K;
-    F[0] = U_K[0] * phi;
-    F[1] = U_K[1] * phi;
-    F[2] = phi * M + Pn;
-    F[3] = U_K[3] * phi;
+    const float Sp = left ? __builtin_fminf(S_L, 0.0f) : __builtin_fmaxf(S_R, 0.0f);
+    const float p_K = left ? 0.0f : pre_R, mn_K = left ? L.mn : R.mn, mt_K = left ? L.mt : R.mt;
+    const float e = (S_star - vn_K) * __builtin_amdgcn_rcpf(S_K - S_star);
+    const float g = Sp != 0.0f ? Sp * e : 0.0f;
+    const float phi = vn_K + g;
+    const double phid = double(phi);
+    Fd[0] = (left ? L.a : R.a) * phid;
+    Fd[1] = (left ? L.b : R.b) * phid;
+    F[0] = g * (S_K * rho_K) + (mn_K * vn_K + p_K);
+    F[1] = mt_K * phi;
 }
 
 // jump = right minus left state, formed by the caller from exact double differences of the
@@ -538,44 +541,58 @@ struct Weno5 {
 // the quadratic with the stencil's cell averages,
 //   p_k(x) = c + u_k x / 2 + t_k (x^2 - 1/12) / 2   (x in cell widths from the cell centre),
 // whose first- and second-difference forms u_k, t_k are exactly the linear forms inside the
-// smoothness indicator beta_k = u_k^2 / 4 + 13 t_k^2 / 12.  With the node-independent
-// factors q_k (alpha_k = d_k q_k) folded in once per stencil, a node costs
+// smoothness indicator beta_k = u_k^2 / 4 + 13 t_k^2 / 12.  The indicators are taken as
+// 4 beta_k = u_k^2 + (13/3) t_k^2 with the regularisation scaled alike (4 eps, and 4e-10 den^2
+// for WENO-Z), which leaves every nonlinear weight as it is.  With the node-independent
+// factors q_k (alpha_k = d_k q_k, up to a common factor) folded in once per stencil, a node
+// costs
 //   c + (sum_k d_k (A u_k q_k + B t_k q_k)) / (sum_k d_k q_k),  A = x/2, B = (x^2 - 1/12)/2,
-// the same value as the tabulated candidates (WENO_H) up to rounding.  Reciprocals: the
-// hardware estimate in float, one Newton step on it in double.
+// the same value as the tabulated candidates (WENO_H) up to rounding.  u_k, t_k come from
+// the stencil's adjacent differences, or straight from the values when the centre is a
+// literal zero (ZC: stencils of deviations from the centre).  Reciprocals: the hardware
+// estimate in float, one Newton step on it in double.
 template <class T>
 WFV_DEV T rcp_mixed(T x) {
     if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
     else return rcp_nr(x);
 }
-template <class T, int SCHEME>
+template <class T, int SCHEME, bool ZC = false>
 struct WenoLin {
     T c, pu[3], pt[3], q[3];
 
     WFV_DEV WenoLin(T a, T b, T cc, T d, T e) : c(cc) {
-        const T u0 = a - T(4) * b + T(3) * cc, t0 = a - T(2) * b + cc;
-        const T u1 = d - b, t1 = b - T(2) * cc + d;
-        const T u2 = T(-3) * cc + T(4) * d - e, t2 = cc - T(2) * d + e;
-        const T b0 = T(0.25) * (u0 * u0) + T(13.0 / 12.0) * (t0 * t0);
-        const T b1 = T(0.25) * (u1 * u1) + T(13.0 / 12.0) * (t1 * t1);
-        const T b2 = T(0.25) * (u2 * u2) + T(13.0 / 12.0) * (t2 * t2);
+        T u0, t0, u1, t1, u2, t2;
+        if constexpr (ZC) {
+            u0 = a - T(4) * b; t0 = a - T(2) * b;
+            u1 = d - b;        t1 = b + d;
+            u2 = T(4) * d - e; t2 = e - T(2) * d;
+        } else {
+            const T d1 = b - a, d2 = cc - b, d3 = d - cc, d4 = e - d;
+            u0 = T(3) * d2 - d1; t0 = d2 - d1;
+            u1 = d2 + d3;        t1 = d3 - d2;
+            u2 = T(3) * d3 - d4; t2 = d4 - d3;
+        }
+        constexpr T K = T(13.0 / 3.0);
+        const T b0 = u0 * u0 + (K * t0) * t0;
+        const T b1 = u1 * u1 + (K * t1) * t1;
+        const T b2 = u2 * u2 + (K * t2) * t2;
         if constexpr (SCHEME == 0) {
             // WENO5threconstruction (include/Utility.hpp:48-61): alpha_k = d_k/(eps+beta_k)^2
-            const T eps = T(1.0e-6);
+            const T eps = T(4.0e-6);
             q[0] = rcp_mixed((eps + b0) * (eps + b0));
             q[1] = rcp_mixed((eps + b1) * (eps + b1));
             q[2] = rcp_mixed((eps + b2) * (eps + b2));
         } else {
             // WENO5Zthreconstruction (include/Utility.hpp:261-280) on the min-max normalised
             // stencil: the indicators are shift-invariant quadratic forms, so tau/(beta_k + eps)
-            // there equals tau_raw/(beta_k,raw + eps den^2) here; the floor (the smallest
-            // normal number) keeps a constant stencil at q = 1 where eps den^2 underflows.
-            // The normalisation's + 1e-20 offset matters only below that floor and is left
-            // out; the max/min chains become 3-operand instructions.
+            // there equals tau_raw/(beta_k,raw + eps den^2) here; the smallest normal number
+            // added keeps a constant stencil at q = 1 where eps den^2 underflows.  The
+            // normalisation's + 1e-20 offset matters only below that floor and is left out;
+            // the max/min chains become 3-operand instructions.
             const T mx = fmax(fmax(fmax(fmax(a, b), cc), d), e);
             const T mn = fmin(fmin(fmin(fmin(a, b), cc), d), e);
             const T den = mx - mn;
-            const T e2 = fmax(T(1.0e-10) * den * den, sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308));
+            const T e2 = (T(4.0e-10) * den) * den + (sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308));
             const T tau = fabs(b0 - b2);
             q[0] = tau * rcp_mixed(b0 + e2) + T(1);
             q[1] = tau * rcp_mixed(b1 + e2) + T(1);
@@ -599,9 +616,9 @@ struct WenoLin {
 };
 
 // the 5-point reconstruction of a policy: the reference's arithmetic in the all-FP64 policy,
-// the linear form in the mixed one
-template <bool EXACT, class T, int SCHEME>
-using WenoSel = typename std::conditional<EXACT, Weno5<T, SCHEME>, WenoLin<T, SCHEME>>::type;
+// the linear form in the mixed one (ZC: the centre value is a literal zero)
+template <bool EXACT, class T, int SCHEME, bool ZC = false>
+using WenoSel = typename std::conditional<EXACT, Weno5<T, SCHEME>, WenoLin<T, SCHEME, ZC>>::type;
 
 // ------------------------------------------------------------------ 7-point kernels
 // WENO-EXP (include/Utility.hpp:291-472) and the THINC kernel of the primitive3 form
