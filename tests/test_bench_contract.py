@@ -64,10 +64,18 @@ def test_accuracy_block_counts_instances_against_tolerance_and_spread():
     j, i = w2.ny // 2, w2.nx // 2
     S[2][2:, j, i] += 1e-3 * np.abs(Sd[2][:2, j, i].sum()) + 1.0
     spread = [s * (1 + 1e-12) for s in Sd]
-    acc = bench.accuracy_block(wls, S, Sd, spread, spread, 500)
+    acc = bench.accuracy_block(wls, S, Sd, spread, [spread, spread], 500)
     pi = acc["per_instance_max_field_rel_l1"]
     assert acc["steps"] == 500 and pi["fraction_within_tolerance"] == pytest.approx(2 / 3)
     assert acc["fraction_within_tolerance_or_fp64_6e-8_spread"] == pytest.approx(2 / 3)
     (b,) = acc["beyond_tolerance_and_spread"]
     assert b["instance"] == 2 and b["field"] in ("u", "v") and (b["where"]["i"], b["where"]["j"]) == (i, j)
+    assert len(b["fp64_6e-8_spread_per_direction"]) == 2
     assert acc["rel_l1_per_field_batch"]["z1rho1"] < 1e-8
+    # an instance's spread is the largest over the perturbation directions: a second direction
+    # that moves instance 2 as far as the mixed state covers it, the first direction alone does not
+    wide = [spread[0], spread[1], S[2].copy()]
+    acc = bench.accuracy_block(wls, S, Sd, spread, [spread, wide], 500)
+    assert acc["fraction_within_tolerance_or_fp64_6e-8_spread"] == pytest.approx(1.0)
+    assert acc["fraction_within_tolerance_or_fp64_6e-8_spread_first_direction"] == pytest.approx(2 / 3)
+    assert acc["beyond_tolerance_and_spread"] == []

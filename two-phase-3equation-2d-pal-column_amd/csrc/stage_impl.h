@@ -183,6 +183,10 @@ template <class W>
 struct WaveLds {
     union {
         W y[16][LANES];          // y-sweep values [g*4+v][lane] (mixed: deviations)
+        struct {
+            double d[8][LANES];  // mixed conservative form: partial densities [g*2+v][lane]
+            W m[8][LANES];       // and momenta [g*2+v-2][lane], as deviations
+        } yh;
         double west[16][LANES];  // limited west face states [v*4+g][lane]
     };
     double base[4][LANES];  // mixed: reconstructed variables of the centre row [v][lane]
@@ -368,6 +372,13 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr int R = ST::R, NW = ST::NW, HALO = ST::HALO, SW = ST::SW;
     constexpr bool EX = PR::exact;
     constexpr bool DEV = !EX && R == 2;
+    // HYB (mixed conservative form): the partial densities are reconstructed in double, the
+    // momenta in W.  The pressure at a face is c_k^2 (rho_k - rho_k0) of the reconstructed
+    // partial densities, so it amplifies their rounding by c_k^2 rho_k / p (about 100 for a
+    // liquid at c = 10, p = 1): an FP32-rounded z1rho1 deviation of O(1) at an interface would
+    // put a 6e-6 relative error into the pressure of every flux there.  The primitive forms
+    // reconstruct the pressure itself and convert back in double, so they keep W throughout.
+    constexpr bool HYB = DEV && RECON == 0 && !std::is_same<W, double>::value;
     constexpr bool TRIM = R == 2 && RECON <= 1;
     // EARLY (mixed policy): a row's height and its RK base state are loaded well before their
     // use, so that their latency overlaps arithmetic; the all-FP64 kernels, whose registers
@@ -563,6 +574,34 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 #pragma unroll
                     for (int v = 0; v < 4; ++v) L.y[g * 4 + v][lane] = yv[v];
                 }
+            } else if constexpr (HYB) {
+                // partial densities in double, momenta in W; deviations from row r (the
+                // centre's own is the literal 0)
+                double sd[5][2];
+                W sm[5][2];
+#pragma unroll
+                for (int t = 0; t < 5; ++t)
+#pragma unroll
+                    for (int v = 0; v < 2; ++v) {
+                        sd[t][v] = t == 2 ? 0.0 : win[t][v] - Bc[v];
+                        sm[t][v] = t == 2 ? W(0) : W(win[t][v + 2] - Bc[v + 2]);
+                    }
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    const double s5[5] = {sd[0][v], sd[1][v], sd[2][v], sd[3][v], sd[4][v]};
+                    double o[4];
+                    recon4<EX, double, SCHEME, true>(s5, o, rows);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) L.yh.d[g * 2 + v][lane] = o[g];
+                }
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    const W s5[5] = {sm[0][v], sm[1][v], sm[2][v], sm[3][v], sm[4][v]};
+                    W o[4];
+                    recon4<EX, W, SCHEME, true>(s5, o, rows);
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) L.yh.m[g * 2 + v][lane] = o[g];
+                }
             } else {
                 W sw[5][4];
 #pragma unroll
@@ -609,6 +648,8 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 else return W(L.base[v][l] - Bc[v]) + L.y[gy * 4 + v][l];
             };
             W ug[4][4][4];  // Ugp[gx][gy][v] at the 12 boundary nodes
+            double ugd[4][4][2];  // HYB: the partial densities' node values [gx][gy][v]
+            W ugm[4][4][2];       // and the momenta's [gx][gy][v-2]
             if constexpr (R == 3 && RECON == 1) {
                 double Lm[16], Rm[16];
                 eigen_LR<double, true, !EX>(P, Uc, Lm, Rm);
@@ -718,6 +759,45 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                             ug[2][gy][v] = st.template node<2>();
                         }
                     }
+            } else if constexpr (HYB) {
+                // variable v of every node row: its y-values ys[gy*2 + v%2] and the neighbours'
+                // bases relative to this cell's (shared by the node rows), in T
+                auto sweep = [&](auto zero, const auto (&ys)[8][LANES], auto (&un)[4][4][2], int v) {
+                    using T = decltype(zero);
+                    T bd[5];
+#pragma unroll
+                    for (int t = 0; t < 5; ++t) bd[t] = t == 2 ? T(0) : T(L.base[v][lane - 2 + t] - Bc[v]);
+#pragma unroll
+                    for (int gy = 0; gy < 4; ++gy) {
+                        T n[4] = {T(0), T(0), T(0), T(0)};
+                        if (rows & (1u << gy)) {
+                            T st5[5];
+#pragma unroll
+                            for (int t = 0; t < 5; ++t) {
+                                // the centre's base is this cell's: its value is its y-deviation
+                                const T y = ys[gy * 2 + (v & 1)][lane - 2 + t];
+                                st5[t] = t == 2 ? y : bd[t] + y;
+                            }
+                            WenoSel<EX, T, SCHEME> st(st5[0], st5[1], st5[2], st5[3], st5[4]);
+                            n[0] = st.template node<0>();
+                            n[3] = st.template node<3>();
+                            if (gy == 0 || gy == 3) {
+                                n[1] = st.template node<1>();
+                                n[2] = st.template node<2>();
+                            }
+                        }
+                        un[0][gy][v & 1] = n[0];
+                        un[3][gy][v & 1] = n[3];
+                        if (gy == 0 || gy == 3) {
+                            un[1][gy][v & 1] = n[1];
+                            un[2][gy][v & 1] = n[2];
+                        }
+                    }
+                };
+#pragma unroll
+                for (int v = 0; v < 2; ++v) sweep(0.0, L.yh.d, ugd, v);
+#pragma unroll
+                for (int v = 2; v < 4; ++v) sweep(W(0), L.yh.m, ugm, v);
             } else {
                 double bc[4] = {Bc[0], Bc[1], Bc[2], Bc[3]};
 #pragma unroll VU
@@ -774,12 +854,25 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     for (int v = 0; v < 4; ++v) f.set(v, g, double(w[v]), Uc[v]);
                 }
             };
+            auto to_face_h = [&](int gx, int gy, FS &f, int g) {
+                f.set_recon(0, g, ugd[gx][gy][0], Uc[0]);
+                f.set_recon(1, g, ugd[gx][gy][1], Uc[1]);
+                f.set_recon(2, g, ugm[gx][gy][0], Uc[2]);
+                f.set_recon(3, g, ugm[gx][gy][1], Uc[3]);
+            };
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                to_face(ug[0][g], fw, g);
-                to_face(ug[3][g], fe, g);
-                to_face(ug[g][0], fs, g);
-                to_face(ug[g][3], fn, g);
+                if constexpr (HYB) {
+                    to_face_h(0, g, fw, g);
+                    to_face_h(3, g, fe, g);
+                    to_face_h(g, 0, fs, g);
+                    to_face_h(g, 3, fn, g);
+                } else {
+                    to_face(ug[0][g], fw, g);
+                    to_face(ug[3][g], fe, g);
+                    to_face(ug[g][0], fs, g);
+                    to_face(ug[g][3], fn, g);
+                }
             }
             if (bp) {
                 // getPositivityPreservingTheta (src/WENOFV.cpp:1936-2000), double in both policies

@@ -7,9 +7,12 @@
 //   PrecMixed : WENO smoothness indicators, nonlinear weights and flux arithmetic in
 //               float with hardware reciprocals; the equation of state (volume
 //               fraction, pressure, sound speed), the limiter's theta and the RK
-//               accumulation stay in double.  The forms that must reconstruct in double
-//               (characteristic, WENO-EXP, primitive3) do so with Newton-refined
+//               accumulation stay in double, and so does the reconstruction of the
+//               partial densities, the equation of state's inputs at the faces (the
+//               conservative form; see stage_body).  The forms that must reconstruct in
+//               double (characteristic, WENO-EXP, primitive3) do so with Newton-refined
 //               reciprocals in place of IEEE divisions.
+//   PrecMixedW64 : the mixed policy with every 5-point reconstruction in double.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -41,7 +44,7 @@ struct PrecMixed {
 };
 // PrecMixedW64 (precision 2): the mixed policy with the 5-point reconstruction in double, as
 // the mixed characteristic and 7-point forms already reconstruct; the Riemann-flux arithmetic
-// stays FP32.  FP32 reconstruction is what sets the mixed policy's error against all-FP64.
+// stays FP32.
 struct PrecMixedW64 {
     using W = double;
     using F = float;
