@@ -227,16 +227,19 @@ def where_worst(fm, fd, w):
             "class": cls}
 
 
-def accuracy_block(wls, S, Sd, S13, S8s, steps):
+def accuracy_block(wls, S, Sd, S13, S8s, steps, Sfma=None):
     """Relative L1 per field of the states S against the all-FP64 states Sd over the batch
     (aggregate) and per instance, beside the FP64 solver's own spread from initial data
     perturbed by 1e-13 (S13) and by 6e-8 (FP32's unit roundoff) in several seeded random
     directions (S8s: one list of states per direction).  An instance's 6e-8 spread is the
-    largest over the directions; the first direction's alone is reported beside it."""
+    largest over the directions; the first direction's alone is reported beside it.  Sfma:
+    the states of the all-FP64 solver compiled with fused multiply-add (precision 3), which
+    differs from the bit-faithful one by rounding alone, at every step; an instance's FP64
+    floor is the largest of its 1e-13 and 6e-8 spreads and that solver's deviation."""
     num = {k: 0.0 for k in FIELDS}
     den = dict(num)
     num13, num8 = dict(num), dict(num)
-    inst, sp13, sp8, sp8a, zb, mass, beyond = [], [], [], [], [1.0, 0.0], 0.0, []
+    inst, sp13, sp8, sp8a, fma, zb, mass, beyond = [], [], [], [], [], [1.0, 0.0], 0.0, []
 
     def gmax(e):
         return max(e[0][k] for k in GATE)
@@ -256,16 +259,19 @@ def accuracy_block(wls, S, Sd, S13, S8s, steps):
         per_dir = [gmax(field_errors(S8[k], Sd[k], w.params)) for S8 in S8s]
         sp8a.append(per_dir[0])
         sp8.append(max(per_dir))
+        fma.append(gmax(field_errors(Sfma[k], Sd[k], w.params)) if Sfma is not None else 0.0)
         if inst[-1] > max(1e-5, sp8[-1]):
             worst = max(GATE, key=lambda q: e[0][q])
             beyond.append({"instance": k, "grid": [w.nx, w.ny], "field": worst, "rel_l1": inst[-1],
                            "fp64_6e-8_spread": sp8[-1], "fp64_6e-8_spread_per_direction": per_dir,
-                           "fp64_1e-13_spread": sp13[-1], "where": where_worst(fm, fd, w)})
+                           "fp64_1e-13_spread": sp13[-1], "fp64_fma_deviation": fma[-1],
+                           "where": where_worst(fm, fd, w)})
         zb = [min(zb[0], float(fm["z1"].min())), max(zb[1], float(fm["z1"].max()))]
         m0 = w.U0[:2].sum(axis=(1, 2))
         mass = max(mass, float(np.abs(S[k][:2].sum(axis=(1, 2)) / m0 - 1).max()))
     agg = {k: float(num[k] / den[k]) for k in FIELDS}
-    inst, sp13, sp8, sp8a = np.array(inst), np.array(sp13), np.array(sp8), np.array(sp8a)
+    inst, sp13, sp8, sp8a, fma = np.array(inst), np.array(sp13), np.array(sp8), np.array(sp8a), np.array(fma)
+    floor = np.maximum.reduce([np.full_like(inst, 1e-5), sp8, sp13, fma])
 
     def dist(x):
         return {"median": float(np.median(x)), "p90": float(np.percentile(x, 90)), "max": float(x.max())}
@@ -285,6 +291,10 @@ def accuracy_block(wls, S, Sd, S13, S8s, steps):
         "fraction_within_tolerance_or_fp64_6e-8_spread_first_direction":
             float((inst <= np.maximum(1e-5, sp8a)).mean()),
         "beyond_tolerance_and_spread": beyond,
+        "fraction_within_tolerance_or_fp64_floor": float((inst <= floor).mean()),
+        "beyond_tolerance_and_fp64_floor": [b["instance"] for b in beyond if b["rel_l1"] > floor[b["instance"]]],
+        "fp64_fma_deviation": (dict(dist(fma), fraction_beyond_tolerance=float((fma > 1e-5).mean()))
+                               if Sfma is not None else None),
         "z1_range": zb, "z1_in_unit_interval": zb[0] >= 0.0 and zb[1] <= 1.0, "max_phase_mass_drift": mass}
 
 
@@ -368,13 +378,14 @@ def run_batch(a, torch, wenofv, cases):
         states = {p: (r["states"] if steps == acc_steps else go(p, acc_steps)["states"]) for p, r in timed_runs.items()}
         S13 = go(0, acc_steps, perturb=1e-13)["states"]
         S8s = [go(0, acc_steps, perturb=6e-8, seed=17 + d)["states"] for d in range(a.spread_directions)]
-        out["accuracy"] = accuracy_block(wls, states[head], states[0], S13, S8s, acc_steps)
+        Sfma = go(3, acc_steps)["states"]
+        out["accuracy"] = accuracy_block(wls, states[head], states[0], S13, S8s, acc_steps, Sfma)
         if other is not None:
             orate = cells * 3 * steps / (timed_runs[other]["ms"] * 1e-3)
             out[f"precision{other}"] = {
                 "precision": pname[other], "value": orate, "ms_per_step": timed_runs[other]["ms"] / steps,
                 "flags": timed_runs[other]["flags"], "speedup_vs_fp64_batched_gpu": orate / fp64_rate,
-                "accuracy": accuracy_block(wls, states[other], states[0], S13, S8s, acc_steps),
+                "accuracy": accuracy_block(wls, states[other], states[0], S13, S8s, acc_steps, Sfma),
                 "note": {1: "FP32 reconstruction and fluxes, FP64 state, EOS, RK update and CFL reduction",
                          2: "FP64 reconstruction, FP32 fluxes; FP64 state, EOS, RK update and CFL "
                             "reduction"}[other]}

@@ -51,8 +51,11 @@ typedef struct wfv_options {
                             2 primitive (z1, rho1, u, v), 3 primitive2 (z1, p, u, v),
                             4 primitive3 (THINC on z1 and WENO-Z on p, u, v; 7-point) */
     int precision;       /* 0: all-FP64, bit-faithful to the reference; 1: mixed precision (FP32
-                            WENO and Riemann fluxes; FP64 equation of state, limiter theta and
-                            RK accumulation); 2: mixed with the reconstruction in FP64 too */
+                            WENO of the momenta and FP32 Riemann fluxes; FP64 reconstruction of
+                            the partial densities, equation of state, limiter theta and RK
+                            accumulation); 2: mixed with every reconstruction in FP64;
+                            3: all-FP64 with fused multiply-add (FP64-accurate, not
+                            bit-faithful: a rounding-sensitivity control; 5-point schemes) */
     int boundary;        /* BOUNDARYTYPE (reference include/Equations.hpp:25-32; its MPI solver
                             runs only 0): 0 periodic; 1 slip walls (NEUMANN, FREESLIP: mirror,
                             normal momentum negated); 2 no-slip walls (NoSlip: mirror, both

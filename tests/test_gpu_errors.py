@@ -32,7 +32,8 @@ def state(nx, ny):
     (dict(recon=5), "RECONSTRUCTIONTYPE"),
     (dict(flux=2), "RIEMANNFLUX"),
     (dict(rk=3), "RKMETH"),
-    (dict(precision=3), "precision"),
+    (dict(precision=4), "precision"),
+    (dict(precision=3, scheme=2), "5-point"),
     (dict(bc=4), "boundary"),
     (dict(cfl=0.0), "CFL"),
     (dict(nx=70, geometry_ranks=4), "divisible"),

@@ -27,6 +27,7 @@ UNITS = [
     ("wenofv_kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("wenofv_kernels_mixed_r3.hip", ["-ffp-contract=fast"]),
     ("wenofv_kernels_mixedw.hip", ["-ffp-contract=fast"]),
+    ("wenofv_kernels_fp64c.hip", ["-ffp-contract=fast"]),
     ("wenofv_control.hip", ["-ffp-contract=off"]),
     ("wenofv_noalloc.hip", ["-ffp-contract=off"]),
     ("wenofv_capi.cpp", ["-ffp-contract=off"]),

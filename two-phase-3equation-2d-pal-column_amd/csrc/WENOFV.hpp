@@ -116,7 +116,7 @@ public:
     RKMETHOD m_rkMeth = RK3;
     RIEMANNFLUXTYPE m_riemannFluxType = LLF;
     RECONSTRUCTIONTYPE m_reconstructionType = CONVERVATIVE;
-    int m_precision = 0;  // PRECISION key: 0 all-FP64 (default), 1 mixed
+    int m_precision = 0;  // PRECISION key: 0 all-FP64 (default), 1 mixed, 2 mixed with FP64 WENO, 3 FP64 + FMA
 
     // initializeSolver (src/WENOFV.cpp:16-230): options, case, grid, mirrors, device context
     void initializeSolver();

@@ -40,8 +40,11 @@ const char *check_options(const wfv_options *opt, const wfv_case *cs) {
     if (opt->scheme < 0 || opt->scheme > 2) return "SCHEME must be 0 (WENO-JS), 1 (WENO-Z) or 2 (WENO-EXP)";
     if (opt->riemann_flux < 0 || opt->riemann_flux > 1) return "RIEMANNFLUX must be 0 or 1";
     if (opt->reconstruction < 0 || opt->reconstruction > 4) return "RECONSTRUCTIONTYPE must be 0..4";
-    if (opt->precision < 0 || opt->precision > 2)
-        return "precision must be 0 (fp64), 1 (mixed) or 2 (mixed, reconstruction in fp64)";
+    if (opt->precision < 0 || opt->precision > 3)
+        return "precision must be 0 (fp64), 1 (mixed), 2 (mixed, reconstruction in fp64) or 3 (fp64 with fused "
+               "multiply-add)";
+    if (opt->precision == 3 && (opt->scheme == 2 || opt->reconstruction == 4))
+        return "precision 3 has the 5-point schemes only (SCHEME 0-1, RECONSTRUCTIONTYPE 0-3)";
     if (opt->boundary < 0 || opt->boundary > 3)
         return "boundary must be 0 (periodic), 1 (slip walls), 2 (no-slip walls) or 3 (fixed ghost cells)";
     if (!(opt->cfl > 0)) return "CFL must be positive";
@@ -626,7 +629,9 @@ int wfv_batch_create(const wfv_options *opt, const wfv_instance *inst, int n, wf
     else if (opt->scheme < 0 || opt->scheme > 2) why = "SCHEME must be 0, 1 or 2";
     else if (opt->riemann_flux < 0 || opt->riemann_flux > 1) why = "RIEMANNFLUX must be 0 or 1";
     else if (opt->reconstruction < 0 || opt->reconstruction > 4) why = "RECONSTRUCTIONTYPE must be 0..4";
-    else if (opt->precision < 0 || opt->precision > 2) why = "precision must be 0, 1 or 2";
+    else if (opt->precision < 0 || opt->precision > 3) why = "precision must be 0, 1, 2 or 3";
+    else if (opt->precision == 3 && (opt->scheme == 2 || opt->reconstruction == 4))
+        why = "precision 3 has the 5-point schemes only (SCHEME 0-1, RECONSTRUCTIONTYPE 0-3)";
     else if (!(opt->cfl > 0)) why = "CFL must be positive";
     for (int i = 0; i < n && why.empty(); ++i) {
         const wfv_instance &I = inst[i];

@@ -195,8 +195,12 @@ hipError_t launch_batch_mixedw(const BatchStageArgs &B, int recon, int scheme, i
 // other forms reconstruct in double under the mixed policy already
 static bool own_mixedw(int recon, int scheme) { return scheme != 2 && (recon == 0 || recon == 2 || recon == 3); }
 
+hipError_t launch_batch_fp64c(const BatchStageArgs &B, int recon, int scheme, int flux, int grid, hipStream_t s);
+hipError_t launch_stage_fp64c(const StageArgs &A, int recon, int scheme, hipStream_t s);
+
 hipError_t launch_batch_stage(const BatchStageArgs &B, int precision, int recon, int scheme, int flux, int grid,
                               hipStream_t s) {
+    if (precision == 3) return launch_batch_fp64c(B, recon, scheme, flux, grid, s);
     if (precision == 2 && own_mixedw(recon, scheme)) return launch_batch_mixedw(B, recon, scheme, flux, grid, s);
     if (precision >= 1) return launch_batch_mixed(B, recon, scheme, flux, grid, s);
     return launch_batch_fp64(B, recon, scheme, flux, grid, s);
@@ -207,6 +211,7 @@ hipError_t launch_stage_mixed(const StageArgs &A, int recon, int scheme, hipStre
 hipError_t launch_stage_mixedw(const StageArgs &A, int recon, int scheme, hipStream_t s);
 
 hipError_t launch_stage(const StageArgs &A, int precision, int recon, int scheme, hipStream_t s) {
+    if (precision == 3) return launch_stage_fp64c(A, recon, scheme, s);
     if (precision == 2 && own_mixedw(recon, scheme)) return launch_stage_mixedw(A, recon, scheme, s);
     if (precision >= 1) return launch_stage_mixed(A, recon, scheme, s);
     return launch_stage_fp64(A, recon, scheme, s);
