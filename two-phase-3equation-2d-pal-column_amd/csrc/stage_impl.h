@@ -61,9 +61,13 @@ WFV_DEV void wave_sync() {
 WFV_DEV int map_index(int i, int n, int bc, bool &mirrored) {
     mirrored = false;
     if (bc == 0) {
-        i %= n;
-        if (i < 0) i += n;
-        return i;
+        // the indices asked for lie within a stencil's reach of the grid (-4 <= i < n + 4) and
+        // n >= 3, so two corrections each way wrap them without an integer division
+        i += i < 0 ? n : 0;
+        i += i < 0 ? n : 0;
+        i -= i >= n ? n : 0;
+        i -= i >= n ? n : 0;
+        return i < 0 ? 0 : (i >= n ? n - 1 : i);
     }
     if (i < 0) { i = -1 - i; mirrored = true; }
     if (i >= n) { i = 2 * n - 1 - i; mirrored = true; }

@@ -579,23 +579,51 @@ struct WenoLin {
         const T b0 = u0 * u0 + (K * t0) * t0;
         const T b1 = u1 * u1 + (K * t1) * t1;
         const T b2 = u2 * u2 + (K * t2) * t2;
+        // In double the q_k are scaled by the product of their three denominators, which the
+        // ratio of a node's q-weighted sums cancels: two products per q_k replace a
+        // Newton-refined reciprocal (three instructions, one of them a transcendental).  The
+        // denominators are at least 1.6e-11 (JS) and 1e-100 (Z) and, for stencil differences
+        // below 1e15, at most 1e61, so their products stay normal.
+        constexpr bool SCALED = sizeof(T) == 8;
         if constexpr (SCHEME == 0) {
             // WENO5threconstruction (include/Utility.hpp:48-61): alpha_k = d_k/(eps+beta_k)^2
             const T eps = T(4.0e-6);
-            q[0] = rcp_mixed((eps + b0) * (eps + b0));
-            q[1] = rcp_mixed((eps + b1) * (eps + b1));
-            q[2] = rcp_mixed((eps + b2) * (eps + b2));
+            const T E0 = (eps + b0) * (eps + b0), E1 = (eps + b1) * (eps + b1), E2 = (eps + b2) * (eps + b2);
+            if constexpr (SCALED) {
+                q[0] = E1 * E2;
+                q[1] = E0 * E2;
+                q[2] = E0 * E1;
+            } else {
+                q[0] = rcp_mixed(E0);
+                q[1] = rcp_mixed(E1);
+                q[2] = rcp_mixed(E2);
+            }
+        } else if constexpr (SCALED) {
+            // WENO5Zthreconstruction as below, q_k = (B_k + tau) prod_{j != k} B_j with
+            // B_k = beta_k + eps den^2; the floor added to the regularisation is 1e-100 (it
+            // matters only for stencils flat to 1e-38, whose node values do not depend on
+            // the weights), so that a constant stencil's q_k = B^3 stays normal
+            const T mx = fmax(fmax(fmax(fmax(a, b), cc), d), e);
+            const T mn = fmin(fmin(fmin(fmin(a, b), cc), d), e);
+            const T den = mx - mn;
+            const T e2 = (T(4.0e-10) * den) * den + T(1.0e-100);
+            const T tau = fabs(b0 - b2);
+            const T B0 = b0 + e2, B1 = b1 + e2, B2 = b2 + e2;
+            const T P01 = B0 * B1, P02 = B0 * B2, P12 = B1 * B2, PI = P01 * B2;
+            q[0] = tau * P12 + PI;
+            q[1] = tau * P02 + PI;
+            q[2] = tau * P01 + PI;
         } else {
             // WENO5Zthreconstruction (include/Utility.hpp:261-280) on the min-max normalised
             // stencil: the indicators are shift-invariant quadratic forms, so tau/(beta_k + eps)
-            // there equals tau_raw/(beta_k,raw + eps den^2) here; the smallest normal number
+            // there equals tau_raw/(beta_k,raw + eps den^2) here; the smallest normal float
             // added keeps a constant stencil at q = 1 where eps den^2 underflows.  The
             // normalisation's + 1e-20 offset matters only below that floor and is left out;
             // the max/min chains become 3-operand instructions.
             const T mx = fmax(fmax(fmax(fmax(a, b), cc), d), e);
             const T mn = fmin(fmin(fmin(fmin(a, b), cc), d), e);
             const T den = mx - mn;
-            const T e2 = (T(4.0e-10) * den) * den + (sizeof(T) == 4 ? T(1.17549435e-38f) : T(2.2250738585072014e-308));
+            const T e2 = (T(4.0e-10) * den) * den + T(1.17549435e-38f);
             const T tau = fabs(b0 - b2);
             q[0] = tau * rcp_mixed(b0 + e2) + T(1);
             q[1] = tau * rcp_mixed(b1 + e2) + T(1);
