@@ -926,8 +926,15 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                         mloc = mn(mloc, fw.get(k, g, uave));
                         mloc = mn(mloc, fe.get(k, g, uave));
                     }
-                    if constexpr (EX) th[k] = scMin(fabs((uave - eps) / (uave - mloc)), 1.0);
-                    else th[k] = scMin(fabs((uave - eps) * rcp_nr(uave - mloc)), 1.0);
+                    if constexpr (EX) {
+                        th[k] = scMin(fabs((uave - eps) / (uave - mloc)), 1.0);
+                    } else {
+                        // the quotient is below 1 only where |uave - eps| < |uave - mloc|; elsewhere
+                        // (a zero or NaN denominator included) theta_k is 1 without a reciprocal
+                        const double nm = uave - eps, dn = uave - mloc;
+                        th[k] = 1.0;
+                        if (fabs(nm) < fabs(dn)) th[k] = scMin(fabs(nm * rcp_nr(dn)), 1.0);
+                    }
                 }
                 theta = scMin(th[0], th[1]);
                 // the limiter flag of the reference's outputs (src/WENOFV.cpp:1906-1909)

@@ -109,17 +109,26 @@ WFV_DEV double sqrt_nr(double x) { return x > 0.0 ? x * rsq_nr(x) : 0.0; }
 // N = dq + S and D = 2 z2rho2 c2^2: z1 = N/(D+N), z2 = D/(D+N) and rho2 = (D+N)/(2 c2^2),
 // so p = c2^2 (rho2 - rho20) = (D+N)/2 - c2^2 rho20 (equal to c1^2 (rho1 - rho10) by the
 // pressure equilibrium the quadratic expresses), and Wood's
-// 1/c^2 = rho (z1^2/(z1rho1 c1^2) + z2^2/(z2rho2 c2^2)) = rho (N^2/(z1rho1 c1^2) + 2D)/(D+N)^2.
-// Only rho, p and c are set.
+// 1/c^2 = rho (z1^2/(z1rho1 c1^2) + z2^2/(z2rho2 c2^2)) = rho (N^2/t1 + 2D)/(D+N)^2 with
+// t1 = z1rho1 c1^2, that is c = (D+N) |t1| / sqrt(t1 rho (N^2 + 2 D t1)): one Newton-refined
+// inverse square root and no reciprocal (|t1| = sqrt(t1^2) keeps the sign of the quotient
+// form where an unlimited reconstruction leaves z1rho1 < 0).  2 D t1 = 4 t1 t2 (t2 =
+// z2rho2 c2^2) is the discriminant's own product term, passed in as f.  Only rho, p and c
+// are set.
+WFV_DEV double wood_c(double rho, double t1, double N, double f, double DN) {
+    return (DN * fabs(t1)) * rsq_nr(t1 * rho * (N * N + f));
+}
 WFV_DEV void eos_fast(const Phys &P, const double U[4], Eos &e) {
     const double a = U[0], b = U[1];
-    const double dq = P.q - (b * P.c2_2 - a * P.c1_2);
-    const double S = sqrt_nr(dq * dq + 4.0 * a * P.c1_2 * b * P.c2_2);
-    const double N = dq + S, D = 2.0 * b * P.c2_2;
-    const double DN = D + N;
+    const double t1 = a * P.c1_2, t2 = b * P.c2_2;
+    const double dq = P.q - (t2 - t1);
+    const double f = 4.0 * t1 * t2;
+    const double S = sqrt_nr(dq * dq + f);
+    const double N = dq + S;
+    const double DN = 2.0 * t2 + N;
     e.rho = a + b;
     e.p = 0.5 * DN - P.c2_2 * P.rho20;
-    e.c = DN * rsq_nr(e.rho * (N * N * rcp_nr(a * P.c1_2) + 2.0 * D));
+    e.c = wood_c(e.rho, t1, N, f, DN);
 }
 
 // The mixed Riemann solvers' equation of state: the pressure as eos_fast and the sound speed
@@ -130,14 +139,15 @@ WFV_DEV void eos_fast(const Phys &P, const double U[4], Eos &e) {
 WFV_DEV void eos_flux(const Phys &P, double a, double b, Eos &e) {
     const double t1 = a * P.c1_2, t2 = b * P.c2_2;
     const double dq = P.q - (t2 - t1);
-    const double x = dq * dq + 4.0 * t1 * t2;
+    const double f = 4.0 * t1 * t2;
+    const double x = dq * dq + f;
     const double y = __builtin_amdgcn_rsq(fmax(x, 1.0e-300));
     const double g = x * y;
-    const double N = g * ((g * y) * -0.5 + 1.5) + dq, D = 2.0 * t2;
-    const double DN = D + N;
+    const double N = g * ((g * y) * -0.5 + 1.5) + dq;
+    const double DN = 2.0 * t2 + N;
     e.rho = a + b;
     e.p = 0.5 * DN - P.c2_2 * P.rho20;
-    e.c = DN * rsq_nr(e.rho * (N * N * rcp_nr(t1) + 2.0 * D));
+    e.c = wood_c(e.rho, t1, N, f, DN);
 }
 
 template <bool FAST>
@@ -328,13 +338,14 @@ WFV_HD void eigen_LR(const Phys &P, const double U[4], T L[16], T R[16]) {
         const double a = U[0], b = U[1];
         const double t1 = a * P.c1_2, t2 = b * P.c2_2;
         const double dq = P.q - (t2 - t1);
-        const double x = dq * dq + 4.0 * t1 * t2;
+        const double f = 4.0 * t1 * t2;
+        const double x = dq * dq + f;
         const double S = x * rsq_nr(fmax(x, 1.0e-300));
         const double N = dq + S, D = 2.0 * t2, DN = D + N;
         const double iDN = rcp_nr(DN);
         const double z1 = N * iDN, z2 = D * iDN;
         const double rho = a + b, irho = rcp_nr(rho);
-        const double c = DN * rsq_nr(rho * (N * N * rcp_nr(t1) + 2.0 * D));
+        const double c = wood_c(rho, t1, N, f, DN);
         const double u = U[2] * irho, v = U[3] * irho;
         const double iz1 = rcp_nr(z1), iz2 = rcp_nr(z2);
         const double zrho1 = t1 * iz1 * iz1, zrho2 = t2 * iz2 * iz2;
