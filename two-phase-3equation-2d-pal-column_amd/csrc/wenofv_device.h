@@ -646,14 +646,31 @@ struct WenoLin {
 
     template <int GP>
     WFV_DEV T node() const {
-        // Gauss-Lobatto nodes x = -1/2, -sqrt(5)/10, sqrt(5)/10, 1/2
-        constexpr double A = GP == 0 ? -0.25 : GP == 1 ? -0.11180339887498948 : GP == 2 ? 0.11180339887498948 : 0.25;
-        constexpr double B = (GP == 0 || GP == 3) ? 1.0 / 12.0 : -1.0 / 60.0;
-        constexpr double d0 = WENO_D[GP][0], d1 = WENO_D[GP][1], d2 = WENO_D[GP][2];
-        const T num = T(d0 * A) * pu[0] + T(d0 * B) * pt[0] + T(d1 * A) * pu[1] + T(d1 * B) * pt[1] +
-                      T(d2 * A) * pu[2] + T(d2 * B) * pt[2];
-        const T sum = T(d0) * q[0] + T(d1) * q[1] + T(d2) * q[2];
-        return c + num * rcp_mixed(sum);
+        // Gauss-Lobatto nodes x = -1/2, -sqrt(5)/10, sqrt(5)/10, 1/2 with A = x/2 and
+        // B = (x^2 - 1/12)/2: the node is c + (A PU + B PT)/S with PU, PT, S the sums of
+        // pu_k, pt_k, q_k weighted by the node's linear weights d_k.  The d_k enter only
+        // through their ratios, so they are scaled to whole numbers at the end nodes
+        // (d = (3, 6, 1)/10 and its mirror) and to a unit centre weight at the inner ones,
+        // and numerator and denominator are divided by B: c + (PT + (A/B) PU)/(S/B) with
+        // A/B = -/+3, 1/B = 12 at the end nodes, A/B = +/-3 sqrt(5), 1/B = -60 inside.  Every
+        // constant is then an exact small integer but three, the inner nodes' d_0/d_1,
+        // d_2/d_1 and 3 sqrt(5).
+        constexpr int k0 = (GP == 0 || GP == 1) ? 0 : 2, k2 = 2 - k0;  // mirrored at GP 2, 3
+        if constexpr (GP == 0 || GP == 3) {
+            const T PU = T(3) * pu[k0] + (T(6) * pu[1] + pu[k2]);
+            const T PT = T(3) * pt[k0] + (T(6) * pt[1] + pt[k2]);
+            const T S = T(3) * q[k0] + (T(6) * q[1] + q[k2]);
+            const T num = GP == 0 ? PT - T(3) * PU : PT + T(3) * PU;
+            return c + num * rcp_mixed(T(12) * S);
+        } else {
+            constexpr double R0 = WENO_D[1][0] / WENO_D[1][1], R2 = WENO_D[1][2] / WENO_D[1][1];
+            constexpr double AL = 6.7082039324993690892;  // 3 sqrt(5)
+            const T PU = T(R0) * pu[k0] + (pu[1] + T(R2) * pu[k2]);
+            const T PT = T(R0) * pt[k0] + (pt[1] + T(R2) * pt[k2]);
+            const T S = T(R0) * q[k0] + (q[1] + T(R2) * q[k2]);
+            const T num = GP == 1 ? PT + T(AL) * PU : PT - T(AL) * PU;
+            return c + num * rcp_mixed(T(-60) * S);
+        }
     }
 };
 
