@@ -651,6 +651,35 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 if constexpr (!DEV) return L.y[gy * 4 + v][l];
                 else return W(L.base[v][l] - Bc[v]) + L.y[gy * 4 + v][l];
             };
+            // node values -> face values of the four sides; left = Ugp[0][g], right = Ugp[3][g],
+            // bottom = Ugp[g][0], top = Ugp[g][3]
+            auto to_face = [&](const W w[4], FS &f, int g) {
+                if constexpr (RECON >= 2) {
+                    double pv[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) pv[v] = DEV ? Bc[v] + double(w[v]) : double(w[v]);
+                    double cv[4];
+                    from_prim<double, double, RECON, !EX>(P, pv, cv);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f.set(v, g, cv[v], Uc[v]);
+                } else if constexpr (DEV || EX) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f.set_recon(v, g, w[v], Uc[v]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f.set(v, g, double(w[v]), Uc[v]);
+                }
+            };
+            // the characteristic forms hand each node's values to its faces as soon as they are
+            // formed (node (gx, gy) lies on the west face if gx = 0, the east if gx = 3, the
+            // south if gy = 0 and the north if gy = 3), so that the 48 node values and the
+            // faces built from them are never live together
+            auto emit = [&](int gx, int gy, const W o[4]) {
+                if (gx == 0) to_face(o, fw, gy);
+                if (gx == 3) to_face(o, fe, gy);
+                if (gy == 0) to_face(o, fs, gx);
+                if (gy == 3) to_face(o, fn, gx);
+            };
             W ug[4][4][4];  // Ugp[gx][gy][v] at the 12 boundary nodes
             double ugd[4][4][2];  // HYB: the partial densities' node values [gx][gy][v]
             W ugm[4][4][2];       // and the momenta's [gx][gy][v-2]
@@ -677,23 +706,12 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     }
 #pragma unroll
                     for (int gx = 0; gx < 4; ++gx) {
+                        if (!all && (gx == 1 || gx == 2)) continue;  // inner nodes of the side faces only
                         double o[4];
                         matvec(Rm, uc[gx], o);
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) rotate_push(ug[gx][v], o[v]);
+                        emit(gx, gy, o);
                     }
                 }
-                // ug[gx][v][gy] -> ug[gx][gy][v]
-#pragma unroll
-                for (int gx = 0; gx < 4; ++gx)
-#pragma unroll
-                    for (int a = 0; a < 4; ++a)
-#pragma unroll
-                        for (int b = a + 1; b < 4; ++b) {
-                            const W t = ug[gx][a][b];
-                            ug[gx][a][b] = ug[gx][b][a];
-                            ug[gx][b][a] = t;
-                        }
             } else if constexpr (R == 3) {
 #pragma unroll VU
                 for (int v = 0; v < 4; ++v) {  // variable v sits in slot 3 after the loop
@@ -713,11 +731,12 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 eigen_LR<W, true, !EX>(P, Uc, Lm, Rm);
 #pragma unroll
                 for (int gy = 0; gy < 4; ++gy) {
+                    const bool all = gy == 0 || gy == 3;
                     if (!(rows & (1u << gy))) {
+                        const W z[4] = {W(0), W(0), W(0), W(0)};
 #pragma unroll
                         for (int gx = 0; gx < 4; ++gx)
-#pragma unroll
-                            for (int v = 0; v < 4; ++v) ug[gx][gy][v] = W(0);
+                            if (all || gx == 0 || gx == 3) emit(gx, gy, z);
                         continue;
                     }
                     W pc[5][4];
@@ -732,18 +751,19 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                         WenoSel<EX, W, SCHEME> st(pc[0][0], pc[1][0], pc[2][0], pc[3][0], pc[4][0]);
                         rotate_push(uc[0], st.template node<0>());
                         rotate_push(uc[3], st.template node<3>());
-                        if (gy == 0 || gy == 3) {
+                        if (all) {
                             rotate_push(uc[1], st.template node<1>());
                             rotate_push(uc[2], st.template node<2>());
                         }
 #pragma unroll
                         for (int t = 0; t < 5; ++t) rotate_left(pc[t]);
                     }
-                    matvec(Rm, uc[0], ug[0][gy]);
-                    matvec(Rm, uc[3], ug[3][gy]);
-                    if (gy == 0 || gy == 3) {
-                        matvec(Rm, uc[1], ug[1][gy]);
-                        matvec(Rm, uc[2], ug[2][gy]);
+#pragma unroll
+                    for (int gx = 0; gx < 4; ++gx) {
+                        if (!all && (gx == 1 || gx == 2)) continue;
+                        W o[4];
+                        matvec(Rm, uc[gx], o);
+                        emit(gx, gy, o);
                     }
                 }
             } else if constexpr (EX) {
@@ -839,25 +859,6 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                     rotate_left(bc);
                 }
             }
-            // node values -> face values of the four sides; left = Ugp[0][g], right = Ugp[3][g],
-            // bottom = Ugp[g][0], top = Ugp[g][3]
-            auto to_face = [&](const W w[4], FS &f, int g) {
-                if constexpr (RECON >= 2) {
-                    double pv[4];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) pv[v] = DEV ? Bc[v] + double(w[v]) : double(w[v]);
-                    double cv[4];
-                    from_prim<double, double, RECON, !EX>(P, pv, cv);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) f.set(v, g, cv[v], Uc[v]);
-                } else if constexpr (DEV || EX) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) f.set_recon(v, g, w[v], Uc[v]);
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) f.set(v, g, double(w[v]), Uc[v]);
-                }
-            };
             auto to_face_h = [&](int gx, int gy, FS &f, int g) {
                 f.set_recon(0, g, ugd[gx][gy][0], Uc[0]);
                 f.set_recon(1, g, ugd[gx][gy][1], Uc[1]);
@@ -866,7 +867,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             };
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                if constexpr (HYB) {
+                if constexpr (RECON == 1) {
+                    // the characteristic branches emitted their faces already
+                } else if constexpr (HYB) {
                     to_face_h(0, g, fw, g);
                     to_face_h(3, g, fe, g);
                     to_face_h(g, 0, fs, g);
