@@ -427,7 +427,13 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     // cell (boundary 3) from the ring or a cell of the state through the periodic or wall
     // map: one load per variable either way, with no branch, so that the window's loads are
     // all in flight together.
-    double win[NW][4];
+    // SLIM (primitive forms): the stencils read the primitive window pwin; of the cell
+    // averages only rows r-1 (the RK completion, win[0]) and r (faces, limiter, RK, win[1])
+    // are kept, and row r is loaded again when it becomes the centre (it entered the window R
+    // iterations earlier, so the caches hold it): the other rows of a second window would
+    // only occupy registers.
+    constexpr bool SLIM = RECON >= 2;
+    double win[SLIM ? 2 : NW][4];
     double pwin[NW][4];
     auto load_row = [&](int row, double out[4]) {
         bool mv;
@@ -444,8 +450,18 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     };
 #pragma unroll
     for (int k = 0; k < NW; ++k) {
-        load_row(y0 - 1 - R + k, win[k]);
-        if constexpr (RECON >= 2) to_prim<double, RECON, !EX>(P, win[k], pwin[k]);
+        if constexpr (SLIM) {
+            double t[4];
+            load_row(y0 - 1 - R + k, t);
+            to_prim<double, RECON, !EX>(P, t, pwin[k]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (k == R - 1) win[0][v] = t[v];
+                if (k == R) win[1][v] = t[v];
+            }
+        } else {
+            load_row(y0 - 1 - R + k, win[k]);
+        }
     }
 
     // The north face values of row r-1 wait in LDS (L.top) for the face below row r; their
@@ -490,30 +506,47 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             !TRIM ? 0xFu
                   : wave_or4(!live ? 0u : (full_l ? 0xFu : ((it == 0 ? 0x8u : 0u) | (it == nrow + 1 ? 0x1u : 0u))));
         if (it > 0) {
+            if constexpr (SLIM) {
 #pragma unroll
-            for (int k = 0; k < NW - 1; ++k)
+                for (int k = 0; k < NW - 1; ++k)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    win[k][v] = win[k + 1][v];
-                    if constexpr (RECON >= 2) pwin[k][v] = pwin[k + 1][v];
+                    for (int v = 0; v < 4; ++v) pwin[k][v] = pwin[k + 1][v];
+                double t[4];
+                if constexpr (PF) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) t[v] = ahead[v];
+                    if (it <= nrow_w) load_row(r + R + 1, ahead);
+                } else {
+                    load_row(r + R, t);
                 }
-            if constexpr (PF) {
+                to_prim<double, RECON, !EX>(P, t, pwin[NW - 1]);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) win[NW - 1][v] = ahead[v];
-                if (it <= nrow_w) load_row(r + R + 1, ahead);
+                for (int v = 0; v < 4; ++v) win[0][v] = win[1][v];
+                load_row(r, win[1]);  // the new centre row's cell averages, again
             } else {
-                load_row(r + R, win[NW - 1]);
+#pragma unroll
+                for (int k = 0; k < NW - 1; ++k)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) win[k][v] = win[k + 1][v];
+                if constexpr (PF) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) win[NW - 1][v] = ahead[v];
+                    if (it <= nrow_w) load_row(r + R + 1, ahead);
+                } else {
+                    load_row(r + R, win[NW - 1]);
+                }
             }
-            if constexpr (RECON >= 2) to_prim<double, RECON, !EX>(P, win[NW - 1], pwin[NW - 1]);
         }
-        const double *Uc = win[R];                          // cell (xc, r)
-        const double *Bc = (RECON >= 2) ? pwin[R] : win[R];  // its reconstructed variables
+        const double *Uc = SLIM ? win[1] : win[R];          // cell (xc, r)
+        const double *Bc = SLIM ? pwin[R] : win[R];         // its reconstructed variables
 
         // ---- 1. y-direction reconstruction of this lane's column at row r
         //      (getFlux_* step 3, src/WENOFV.cpp:670-687; characteristic 796-833)
         {
             auto sv = [&](int t, int v) -> W {  // stencil value (DEV: deviation from row r)
-                const double x = (RECON >= 2) ? pwin[t][v] : win[t][v];
+                double x;
+                if constexpr (SLIM) x = pwin[t][v];
+                else x = win[t][v];
                 if constexpr (!DEV) return x;
                 else return W(x - Bc[v]);
             };
@@ -1056,7 +1089,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 const int rr = r - 1;
                 const double dyp = EARLY ? dy_prev : ((const WFV_GLOBAL double *)A.dyC)[rr];
                 const double idyp = idy_prev;
-                const double *Ucur = win[R - 1];
+                const double *Ucur = SLIM ? win[0] : win[R - 1];
                 if (!A.Un) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) Ub[v] = Ucur[v];
