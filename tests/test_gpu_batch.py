@@ -1,5 +1,5 @@
 """Batched solver: every instance of a batch reproduces a single-domain solver with the
-same options bit for bit (tolerance 0), in both precision policies, with uneven grids,
+same options bit for bit (tolerance 0), in every precision, with uneven grids,
 periodic and wall boundaries and per-instance time steps; in all-FP64 every instance also
 equals the oracle run on its own (state and final time, bit for bit)."""
 import numpy as np
@@ -22,7 +22,7 @@ def make_instances(seed, n):
     return out
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
 @pytest.mark.parametrize("rk,recon", [(2, 0), (1, 1), (0, 3)])
 def test_batch_instances_equal_single_solvers(precision, rk, recon):
     wls = make_instances(11 + rk, 5)
