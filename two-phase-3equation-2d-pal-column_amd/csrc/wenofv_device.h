@@ -768,20 +768,37 @@ struct WenoExp7 {
     const double *s;
     double dx4, a0, a1, a2, l_lo, l_hi;
     WFV_HD WenoExp7(const double *st, double dx4_) : s(st), dx4(dx4_) {
-        double n[5];
-        wexp_norm<FAST>(s, n);
         const double theta = 1;
-        const double b0 = theta * 1.0 / 2.0 * fabs(n[0] - 4 * n[1] + 3 * n[2]) + fabs(n[0] - 2 * n[1] + n[2]);
-        const double b1 = theta * 1.0 / 2.0 * fabs(n[1] - n[3]) + fabs(n[1] - 2 * n[2] + n[3]);
-        const double b2 = theta * 1.0 / 2.0 * fabs(3 * n[2] - 4 * n[3] + n[4]) + fabs(n[2] - 2 * n[3] + n[4]);
-        const double tau = n[0] - 4.0 * n[1] + 6.0 * n[2] - 4.0 * n[3] + n[4];
         const double eps = 1e-16;
         if constexpr (FAST) {
+            // The indicators and tau are differences, of degree 1 in the stencil: on the values
+            // less their minimum (the reference's numerators) they are den times the
+            // normalised ones (den = max - min + 1e-20), so a_k = 1 + 2 tau^2 / (b_k^2 + eps)
+            // keeps its value with eps den^2 in place of eps, and the division by den goes.
+            // The a_k are then scaled by the product of their three denominators (>= 1e-56
+            // each), which the node's weight ratio cancels.
+            const double mx = fmax(fmax(fmax(fmax(s[1], s[2]), s[3]), s[4]), s[5]);
+            const double mn = fmin(fmin(fmin(fmin(s[1], s[2]), s[3]), s[4]), s[5]);
+            const double den = mx - mn + 1e-20;
+            const double e2 = (eps * den) * den;
+            const double u[5] = {s[1] - mn, s[2] - mn, s[3] - mn, s[4] - mn, s[5] - mn};
+            const double b0 = 0.5 * fabs(u[0] - 4 * u[1] + 3 * u[2]) + fabs(u[0] - 2 * u[1] + u[2]);
+            const double b1 = 0.5 * fabs(u[1] - u[3]) + fabs(u[1] - 2 * u[2] + u[3]);
+            const double b2 = 0.5 * fabs(3 * u[2] - 4 * u[3] + u[4]) + fabs(u[2] - 2 * u[3] + u[4]);
+            const double tau = u[0] - 4.0 * u[1] + 6.0 * u[2] - 4.0 * u[3] + u[4];
             const double t2 = 2 * (tau * tau);
-            a0 = 1.0 + t2 * rcp_nr((b0 * b0) + eps);
-            a1 = 1.0 + t2 * rcp_nr((b1 * b1) + eps);
-            a2 = 1.0 + t2 * rcp_nr((b2 * b2) + eps);
+            const double E0 = b0 * b0 + e2, E1 = b1 * b1 + e2, E2 = b2 * b2 + e2;
+            const double P01 = E0 * E1, PI = P01 * E2;
+            a0 = t2 * (E1 * E2) + PI;
+            a1 = t2 * (E0 * E2) + PI;
+            a2 = t2 * P01 + PI;
         } else {
+            double n[5];
+            wexp_norm<FAST>(s, n);
+            const double b0 = theta * 1.0 / 2.0 * fabs(n[0] - 4 * n[1] + 3 * n[2]) + fabs(n[0] - 2 * n[1] + n[2]);
+            const double b1 = theta * 1.0 / 2.0 * fabs(n[1] - n[3]) + fabs(n[1] - 2 * n[2] + n[3]);
+            const double b2 = theta * 1.0 / 2.0 * fabs(3 * n[2] - 4 * n[3] + n[4]) + fabs(n[2] - 2 * n[3] + n[4]);
+            const double tau = n[0] - 4.0 * n[1] + 6.0 * n[2] - 4.0 * n[3] + n[4];
             a0 = 1.0 + 2 * (tau * tau) / ((b0 * b0) + eps);
             a1 = 1.0 + 2 * (tau * tau) / ((b1 * b1) + eps);
             a2 = 1.0 + 2 * (tau * tau) / ((b2 * b2) + eps);
@@ -810,8 +827,20 @@ struct ExpThinc7 {
         thinc = (1e-6 < uc) & (uc < 1 - 1e-6);
         l_lo = wexp_l2h2<FAST>(s, dx4, false);
         l_hi = wexp_l2h2<FAST>(s + 1, dx4, false);
+        // FAST: the indicators on the values less their minimum (den^2 times the normalised
+        // ones) with eps den^2 in place of eps, and the a_k scaled by the product of their
+        // denominators (>= 1e-56 each), as in WenoExp7
         double n[5];
-        wexp_norm<FAST>(s, n);
+        double e2 = 0.0;
+        if constexpr (FAST) {
+            const double mx = fmax(fmax(fmax(fmax(s[1], s[2]), s[3]), s[4]), s[5]);
+            const double mn = fmin(fmin(fmin(fmin(s[1], s[2]), s[3]), s[4]), s[5]);
+            const double den = mx - mn + 1e-20;
+            e2 = (1.0e-16 * den) * den;
+            for (int k = 0; k < 5; ++k) n[k] = s[1 + k] - mn;
+        } else {
+            wexp_norm<FAST>(s, n);
+        }
         double t;
         t = n[0] - 4 * n[1] + 3 * n[2];
         double b0 = 0.25 * (t * t);
@@ -828,9 +857,11 @@ struct ExpThinc7 {
         const double eps = 1.0e-16;
         const double tau = fabs(b0 - b2);
         if constexpr (FAST) {
-            a0 = 1 + tau * rcp_nr(b0 + eps);
-            a1 = 1 + tau * rcp_nr(b1 + eps);
-            a2 = 1 + tau * rcp_nr(b2 + eps);
+            const double E0 = b0 + e2, E1 = b1 + e2, E2 = b2 + e2;
+            const double P01 = E0 * E1, PI = P01 * E2;
+            a0 = tau * (E1 * E2) + PI;
+            a1 = tau * (E0 * E2) + PI;
+            a2 = tau * P01 + PI;
         } else {
             a0 = 1 + tau / (b0 + eps);
             a1 = 1 + tau / (b1 + eps);
