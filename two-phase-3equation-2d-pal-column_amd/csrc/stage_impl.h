@@ -537,8 +537,15 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 }
             }
         }
-        const double *Uc = SLIM ? win[1] : win[R];          // cell (xc, r)
-        const double *Bc = SLIM ? pwin[R] : win[R];         // its reconstructed variables
+        const double *Uc;  // cell (xc, r)
+        const double *Bc;  // its reconstructed variables
+        if constexpr (SLIM) {
+            Uc = win[1];
+            Bc = pwin[R];
+        } else {
+            Uc = win[R];
+            Bc = win[R];
+        }
 
         // ---- 1. y-direction reconstruction of this lane's column at row r
         //      (getFlux_* step 3, src/WENOFV.cpp:670-687; characteristic 796-833)
@@ -1089,7 +1096,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 const int rr = r - 1;
                 const double dyp = EARLY ? dy_prev : ((const WFV_GLOBAL double *)A.dyC)[rr];
                 const double idyp = idy_prev;
-                const double *Ucur = SLIM ? win[0] : win[R - 1];
+                const double *Ucur;
+                if constexpr (SLIM) Ucur = win[0];
+                else Ucur = win[R - 1];
                 if (!A.Un) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) Ub[v] = Ucur[v];
