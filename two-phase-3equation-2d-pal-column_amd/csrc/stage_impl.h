@@ -1160,6 +1160,18 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     if (lane == 0) {
         if (err && A.flags) atomicOr(A.flags, err);
         if (A.dtmin && dtloc < 1.0e300) atomicMin(A.dtmin, (unsigned long long)__double_as_longlong(dtloc));
+        // Fused step control: every wavefront counts itself out once its error bits and CFL
+        // minimum are in (release); the last of the launch's sp.total wavefronts (acquire) runs
+        // the next step's control, in place of a launch of its own, and re-arms the count.
+        // A wavefront that returned at the top (loop finished) took no part: then none did.
+        if (A.ctl && __hip_atomic_fetch_add(&A.ctl->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                         (unsigned)sp.total - 1u) {
+            const unsigned long long bits =
+                __hip_atomic_load(&A.ctl->dtmin_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned fl = __hip_atomic_load(&A.ctl->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            step_control(A.ctl, bits, fl);
+            A.ctl->done = 0;
+        }
     }
     wave_sync();  // the next strip of a persistent wavefront reuses the LDS
 }

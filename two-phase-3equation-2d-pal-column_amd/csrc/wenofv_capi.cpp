@@ -93,7 +93,8 @@ StageArgs stage_base(const wfv_options &opt, const wfv_case &cs, int nx, int ny,
 }
 
 hipError_t enqueue_rk_step(const StageArgs &base, double *const reg[3], const wfv_options &opt, double dt,
-                           const double *dt_dev, const int *active, unsigned long long *dtmin, hipStream_t s) {
+                           const double *dt_dev, const int *active, unsigned long long *dtmin, hipStream_t s,
+                           StepCtl *fuse_ctl) {
     StagePlan pl[3];
     const int ns = rk_plan(opt.rk_method, pl);
     for (int st = 0; st < ns; ++st) {
@@ -108,6 +109,7 @@ hipError_t enqueue_rk_step(const StageArgs &base, double *const reg[3], const wf
         A.dt_ptr = dt_dev;
         A.active = active;
         A.dtmin = (st == ns - 1 && opt.rk_method != 0) ? dtmin : nullptr;
+        A.ctl = (st == ns - 1 && opt.rk_method != 0 && dtmin) ? fuse_ctl : nullptr;
         hipError_t e = launch_stage(A, opt.precision, opt.reconstruction, opt.scheme, s);
         if (e != hipSuccess) return e;
     }
@@ -162,12 +164,17 @@ int rk_plan(int rk, StagePlan *s) {
 
 namespace {
 
-// enqueue one RK step; dt from the host (dt_dev == nullptr) or from the device loop
-int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active, bool want_dtmin) {
+// enqueue one RK step; dt from the host (dt_dev == nullptr) or from the device loop; fuse: the
+// step's last stage also runs the next step's control (fused_control)
+int enqueue_step(wfv_ctx *c, double dt, const double *dt_dev, const int *active, bool want_dtmin, bool fuse = false) {
     HIPCHK(c, enqueue_rk_step(base_args(c), c->reg, c->opt, dt, dt_dev, active,
-                              want_dtmin ? &c->d_ctl->dtmin_bits : nullptr, c->stream));
+                              want_dtmin ? &c->d_ctl->dtmin_bits : nullptr, c->stream, fuse ? c->d_ctl : nullptr));
     return WFV_OK;
 }
+
+// whether a device loop's steps run their control in the last stage (every scheme whose last
+// stage reduces the CFL minimum: SSP-RK2 and RK3) rather than as a launch before each step
+bool fused_control(const wfv_ctx *c) { return c->opt.rk_method != 0; }
 
 }  // namespace
 
@@ -405,10 +412,15 @@ int advance_impl(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int 
     h.dts = dts ? c->d_dts : nullptr;
     h.max_record = dts ? max_dts : 0;
     h.max_count = fixed_steps > 0 ? 0 : max_count;
+    h.done = 0;
     HIPCHK(c, hipMemcpy(c->d_ctl, &h, sizeof h, hipMemcpyHostToDevice));
     // CFL minimum of the initial state
     DtArgs D{c->reg[0], c->P, c->dX, c->dY, c->opt.cfl, c->nx, c->ny, &c->d_ctl->flags, &c->d_ctl->dtmin_bits};
     HIPCHK(c, launch_dt(D, c->stream));
+    // The control of the first step; every later one runs in the last wavefront of the
+    // previous step's last stage (fused_control), or as a launch before its step.
+    const bool fuse = fused_control(c);
+    if (fuse) HIPCHK(c, launch_step_control(c->d_ctl, c->stream));
     // Steps are replayed from a graph of GRAPH_STEPS steps captured once per context; a
     // step enqueued after the loop has finished returns at once (device flag `active`).
     // A fixed-step run enqueues its remainder as individual launches, so it never runs past
@@ -419,8 +431,8 @@ int advance_impl(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int 
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = WFV_OK;
         for (int k = 0; k < GRAPH_STEPS && rc == WFV_OK; ++k) {
-            if (launch_step_control(c->d_ctl, c->stream) != hipSuccess) rc = WFV_ERR_HIP;
-            if (rc == WFV_OK) rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true);
+            if (!fuse && launch_step_control(c->d_ctl, c->stream) != hipSuccess) rc = WFV_ERR_HIP;
+            if (rc == WFV_OK) rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true, fuse);
         }
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc != WFV_OK) return rc;
@@ -437,8 +449,8 @@ int advance_impl(wfv_ctx *c, double t_start, double t_end, int fixed_steps, int 
         const int full = nfix / GRAPH_STEPS, rest = nfix % GRAPH_STEPS;
         for (int k = 0; k < full; ++k) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
         for (int k = 0; k < rest; ++k) {
-            HIPCHK(c, launch_step_control(c->d_ctl, c->stream));
-            int rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true);
+            if (!fuse) HIPCHK(c, launch_step_control(c->d_ctl, c->stream));
+            int rc = enqueue_step(c, 0.0, &c->d_ctl->dt, &c->d_ctl->active, true, fuse);
             if (rc) return rc;
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));

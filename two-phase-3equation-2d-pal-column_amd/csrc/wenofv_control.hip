@@ -49,30 +49,12 @@ __global__ __launch_bounds__(DT_THREADS) void dt_kernel(DtArgs A) {
     }
 }
 
-// Device-side step control of the reference time loop (src/WENOFV.cpp:318-332):
-// dt = min(timestep, T - now) with timestep = min(1.0, CFL minimum); the loop stops when
-// |now - T| <= 1e-10 (or after max_count steps, where the caller writes output), or after a
-// fixed number of unclipped steps.  It writes the dt the following
-// stages read, advances `now`, and re-arms the CFL minimum for the last stage.
+// the step control (step_control) as a kernel of its own: before the first step of a device
+// loop, and before every step where the last stage does not run it (SSP-RK1, the
+// allocation-free loop)
 __global__ void step_control_kernel(StepCtl *S) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (S->active == 0) return;
-    bool go;
-    if (S->fixed_steps > 0) go = S->count < S->fixed_steps;
-    else go = fabs(S->now - S->t_end) > 1e-10 && (S->max_count <= 0 || S->count < S->max_count);
-    if (!go || (S->flags & WFV_FLAG_NAN_DT)) {
-        S->active = 0;
-        return;
-    }
-    double timestep = 1.0;
-    const double m = __longlong_as_double((long long)S->dtmin_bits);
-    if (S->dtmin_bits != ~0ull && !(m > timestep)) timestep = m;
-    if (S->fixed_steps <= 0) timestep = stdMin(timestep, S->t_end - S->now);
-    S->dt = timestep;
-    if (S->count < S->max_record && S->dts) S->dts[S->count] = timestep;
-    S->now += timestep;
-    S->count += 1;
-    S->dtmin_bits = ~0ull;
+    step_control(S, S->dtmin_bits, S->flags);
 }
 
 // step control of every instance of a batch (one thread per instance)

@@ -83,8 +83,11 @@ StageArgs stage_base(const wfv_options &opt, const wfv_case &cs, int nx, int ny,
 // enqueue the stages of one RK step from reg[0] over reg[1], reg[2] (SSP-RK1: reg[1] copied
 // back into reg[0]); dt from the host (dt_dev == nullptr) or the device loop (dt_dev,
 // active); dtmin (optional) receives the CFL minimum of the new state
+// fuse_ctl (optional, SSP-RK2/3 with dtmin): the last stage's last wavefront runs the next
+// step's control on it (fused step control)
 hipError_t enqueue_rk_step(const StageArgs &base, double *const reg[3], const wfv_options &opt, double dt,
-                           const double *dt_dev, const int *active, unsigned long long *dtmin, hipStream_t s);
+                           const double *dt_dev, const int *active, unsigned long long *dtmin, hipStream_t s,
+                           StepCtl *fuse_ctl = nullptr);
 
 // a padded state [4][ny+2G][nx+2G] -> its ghost ring, SoA [4][ring_size(nx, ny)]
 inline void pack_ghost_ring(int nx, int ny, const double *Upad, double *ring) {

@@ -163,6 +163,8 @@ struct StageArgs {
     StripPlan sp;         // wavefronts of the grid (strip_plan)
     unsigned *flags;                 // error bits (WFV_FLAG_*)
     unsigned long long *dtmin;       // if set: CFL minimum of the output state (bit pattern)
+    struct StepCtl *ctl;             // if set (last stage of a device-loop step): the last
+                                     // wavefront to finish runs the next step's control
     double *faces_out;               // parity hook: limited face values [cell][side][v][g]
     double *theta_out;               // parity hook: limiter theta per cell
     int faces_ring;                  // hooks over [ny+2][nx+2]: the cells and their ghost ring
@@ -192,7 +194,34 @@ struct StepCtl {
     double *dts;                    // optional record of every dt (device memory)
     int max_record;
     int max_count;                  // time-driven loop: stop after this many steps (0: no cap)
+    unsigned done;                  // wavefronts of the step's last stage that have finished
 };
+
+// Device-side step control of the reference time loop (src/WENOFV.cpp:318-332):
+// dt = min(timestep, T - now) with timestep = min(1.0, CFL minimum); the loop stops when
+// |now - T| <= 1e-10 (or after max_count steps, where the caller writes output), or after a
+// fixed number of unclipped steps.  It writes the dt the following stages read, advances
+// `now`, and re-arms the CFL minimum for the last stage.  dtmin_bits and flags are passed in
+// by the caller, which reads them as its memory ordering requires.
+WFV_DEV void step_control(StepCtl *S, unsigned long long dtmin_bits, unsigned flags) {
+    if (S->active == 0) return;
+    bool go;
+    if (S->fixed_steps > 0) go = S->count < S->fixed_steps;
+    else go = fabs(S->now - S->t_end) > 1e-10 && (S->max_count <= 0 || S->count < S->max_count);
+    if (!go || (flags & WFV_FLAG_NAN_DT)) {
+        S->active = 0;
+        return;
+    }
+    double timestep = 1.0;
+    const double m = __longlong_as_double((long long)dtmin_bits);
+    if (dtmin_bits != ~0ull && !(m > timestep)) timestep = m;
+    if (S->fixed_steps <= 0) timestep = stdMin(timestep, S->t_end - S->now);
+    S->dt = timestep;
+    if (S->count < S->max_record && S->dts) S->dts[S->count] = timestep;
+    S->now += timestep;
+    S->count += 1;
+    S->dtmin_bits = ~0ull;
+}
 
 // one instance of a batch (device memory)
 struct BatchInst {
