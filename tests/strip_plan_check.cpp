@@ -4,6 +4,7 @@
 // segment lies inside its wavefront; and every lane an owned cell depends on (the x-stencil
 // lanes of the reconstructions on both sides of its faces, the lanes solving its east and
 // west faces) sits in the same segment, on the adjacent column, with the same rows.
+// Also the autotuner's occupancy candidates and the XCD-aware strip order.
 // Prints "ok <cases>" or the first failure and exits 1.
 #include <cstdio>
 #include <vector>
@@ -82,8 +83,29 @@ static int check_occupancy() {
     return 0;
 }
 
+// The XCD-aware strip order (xcd_strip): a bijection of [0, n) under which the workgroups of
+// each XCD (b mod 8) take one contiguous run of strips in dispatch order, the runs' lengths
+// differing by at most one.
+static int check_xcd_order() {
+    for (int n = 1; n <= 20000; n += n < 600 ? 1 : 37) {
+        std::vector<int> seen(n, 0), first(8, -1), last(8, -1), len(8, 0);
+        for (int b = 0; b < n; ++b) {
+            const int s = wfv::xcd_strip(b, n), x = b % 8;
+            if (s < 0 || s >= n || seen[s]++) return fail("xcd order: not a bijection", n, 0, 0, 0, b, s);
+            if (last[x] >= 0 && s != last[x] + 1) return fail("xcd order: run not contiguous", n, 0, 0, 0, b, s);
+            if (first[x] < 0) first[x] = s;
+            last[x] = s;
+            ++len[x];
+        }
+        for (int x = 0; x < 8; ++x)
+            if (len[x] != n / 8 + (x < n % 8 ? 1 : 0)) return fail("xcd order: run length", n, 0, 0, 0, x, len[x]);
+    }
+    return 0;
+}
+
 int main() {
     if (check_occupancy()) return 1;
+    if (check_xcd_order()) return 1;
     const int nxs[] = {1, 5, 6, 7, 16, 20, 25, 26, 27, 31, 32, 50, 56, 57, 58, 59, 63, 64, 70, 100,
                        112, 115, 116, 117, 128, 170, 174, 200, 256, 300, 512, 1000, 1024};
     const int nys[] = {1, 2, 3, 7, 16, 33, 64, 100, 128, 257};

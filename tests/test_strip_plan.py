@@ -3,8 +3,10 @@ compiles the kernels' own strip_plan / segment_of (hipcc, host code only) and ve
 35,640 combinations of grid shape, rows per strip (every value up to 40 and the larger ones
 the autotuner tries) and stencil halo, that every cell is owned exactly once and that every
 lane a cell's reconstruction and face fluxes read stays in its segment, on the adjacent
-column, with the same rows, under the host sanitizers; and that the autotuner's occupancy
-candidates for a 1024 x 512 grid on 256 CUs are the rows per strip that fill whole rounds."""
+column, with the same rows, under the host sanitizers; that the autotuner's occupancy
+candidates for a 1024 x 512 grid on 256 CUs are the rows per strip that fill whole rounds; and
+that the XCD-aware strip order (xcd_strip) is a bijection giving each XCD one contiguous run
+of strips."""
 import os
 import shutil
 import subprocess

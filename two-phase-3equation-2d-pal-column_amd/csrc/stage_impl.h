@@ -1176,21 +1176,6 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     wave_sync();  // the next strip of a persistent wavefront reuses the LDS
 }
 
-#ifndef WFV_XCD_ORDER
-#define WFV_XCD_ORDER 1
-#endif
-// XCD-aware strip order.  The hardware hands workgroup b to XCD b mod 8, each XCD with its own
-// L2.  Strips are numbered along row bands, so workgroup b takes strip xcd_strip(b, n): each XCD
-// runs one contiguous run of whole row bands, and the rows that vertically adjacent strips
-// share stay in that XCD's L2 (with b itself as the strip, every XCD would see every eighth
-// strip and fetch the shared rows from HBM again).  A bijection of [0, n).
-WFV_DEV int xcd_strip(int b, int n) {
-    if (!WFV_XCD_ORDER) return b;
-    constexpr int NXCD = 8;
-    const int q = n / NXCD, r = n % NXCD, x = b % NXCD;
-    return x * q + (x < r ? x : r) + b / NXCD;
-}
-
 // single domain: one wavefront per strip
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {

@@ -82,6 +82,22 @@ __host__ __device__ inline StripPlan strip_plan(int nx, int ny, int ty, int halo
     return p;
 }
 
+#ifndef WFV_XCD_ORDER
+#define WFV_XCD_ORDER 1
+#endif
+// XCD-aware strip order of the single-domain stage kernels.  The hardware hands workgroup b
+// to XCD b mod 8, each XCD with its own L2.  Strips are numbered along row bands, so
+// workgroup b takes strip xcd_strip(b, n): each XCD runs one contiguous run of whole row
+// bands, and the rows that vertically adjacent strips share stay in that XCD's L2 (with b
+// itself as the strip, every XCD would see every eighth strip and fetch the shared rows from
+// HBM again).  A bijection of [0, n).
+__host__ __device__ inline int xcd_strip(int b, int n) {
+    if (!WFV_XCD_ORDER) return b;
+    constexpr int NXCD = 8;
+    const int q = n / NXCD, r = n % NXCD, x = b % NXCD;
+    return x * q + (x < r ? x : r) + b / NXCD;
+}
+
 // Rows-per-strip candidates that fit a grid's wavefronts to whole rounds of the device.  A
 // grid of a few hundred thousand to a few million cells gives fewer wavefronts than the
 // device holds at once, or a last round that is nearly empty, and a launch lasts as long as
