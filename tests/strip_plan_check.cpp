@@ -4,11 +4,12 @@
 // segment lies inside its wavefront; and every lane an owned cell depends on (the x-stencil
 // lanes of the reconstructions on both sides of its faces, the lanes solving its east and
 // west faces) sits in the same segment, on the adjacent column, with the same rows.
-// Also the autotuner's occupancy candidates and the XCD-aware strip order.
+// Also the autotuner's occupancy candidates and the XCD-aware strip and batch item orders.
 // Prints "ok <cases>" or the first failure and exits 1.
 #include <cstdio>
 #include <vector>
 
+#include "wenofv_host.h"
 #include "wenofv_kernels.h"
 
 using wfv::Segment;
@@ -103,9 +104,65 @@ static int check_xcd_order() {
     return 0;
 }
 
+// The batch's XCD-aware item order (batch_item_order) over seeded random instance lists: every
+// (instance, strip) item exactly once; each XCD's items (positions x, x+8, ...) a prefix
+// without gaps, in instance-priority then strip order; the XCDs' work within one run of
+// the largest per-strip work of the smallest load.
+static int check_batch_order() {
+    unsigned seed = 12345u;
+    auto rnd = [&](int n) { seed = seed * 1664525u + 1013904223u; return (int)((seed >> 8) % (unsigned)n); };
+    const int rows_set[] = {1, 2, 4, 12, 48, 64, 96, 128};
+    for (int trial = 0; trial < 200; ++trial) {
+        const int n = 1 + rnd(300), chunk = 1 + rnd(32);
+        std::vector<int> nstrip(n), rows(n), prio(n);
+        for (int i = 0; i < n; ++i) {
+            nstrip[i] = 1 + rnd(trial % 3 == 0 ? 20 : 600);
+            rows[i] = rows_set[rnd(8)];
+            prio[i] = i;
+        }
+        std::stable_sort(prio.begin(), prio.end(), [&](int a, int b) { return rows[a] > rows[b]; });
+        std::vector<int> rank(n);
+        for (int r = 0; r < n; ++r) rank[prio[r]] = r;
+        std::vector<int> ii, ss;
+        const int P = wfv::batch_item_order(prio, nstrip, rows, chunk, ii, ss);
+        if (P % 8 != 0 || (int)ii.size() != P || (int)ss.size() != P) return fail("batch order: size", n, chunk, P, 0, trial, 0);
+        std::vector<std::vector<char>> seen(n);
+        for (int i = 0; i < n; ++i) seen[i].assign(nstrip[i], 0);
+        long long total = 0;
+        double load[8] = {}, maxw = 0;
+        for (int x = 0; x < 8; ++x) {
+            bool ended = false;
+            int prank = -1, ps = -1;
+            for (int p = x; p < P; p += 8) {
+                const int i = ii[p], st = ss[p];
+                if (i < 0) { ended = true; continue; }
+                if (ended) return fail("batch order: gap before an item", n, chunk, P, 0, trial, p);
+                if (i >= n || st < 0 || st >= nstrip[i] || seen[i][st]++) return fail("batch order: item", n, chunk, P, 0, trial, p);
+                if (rank[i] < prank || (rank[i] == prank && st <= ps)) return fail("batch order: order", n, chunk, P, 0, trial, p);
+                prank = rank[i];
+                ps = st;
+                load[x] += rows[i] + 1.0;
+                maxw = rows[i] + 1.0 > maxw ? rows[i] + 1.0 : maxw;
+                ++total;
+            }
+        }
+        long long want = 0;
+        for (int i = 0; i < n; ++i) want += nstrip[i];
+        if (total != want) return fail("batch order: count", n, chunk, P, 0, trial, (int)total);
+        double lo = load[0], hi = load[0];
+        for (int x = 1; x < 8; ++x) {
+            lo = load[x] < lo ? load[x] : lo;
+            hi = load[x] > hi ? load[x] : hi;
+        }
+        if (hi - lo > chunk * maxw + 1e-9) return fail("batch order: balance", n, chunk, P, 0, trial, (int)(hi - lo));
+    }
+    return 0;
+}
+
 int main() {
     if (check_occupancy()) return 1;
     if (check_xcd_order()) return 1;
+    if (check_batch_order()) return 1;
     const int nxs[] = {1, 5, 6, 7, 16, 20, 25, 26, 27, 31, 32, 50, 56, 57, 58, 59, 63, 64, 70, 100,
                        112, 115, 116, 117, 128, 170, 174, 200, 256, 300, 512, 1000, 1024};
     const int nys[] = {1, 2, 3, 7, 16, 33, 64, 100, 128, 257};

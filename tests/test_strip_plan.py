@@ -6,7 +6,9 @@ lane a cell's reconstruction and face fluxes read stays in its segment, on the a
 column, with the same rows, under the host sanitizers; that the autotuner's occupancy
 candidates for a 1024 x 512 grid on 256 CUs are the rows per strip that fill whole rounds; and
 that the XCD-aware strip order (xcd_strip) is a bijection giving each XCD one contiguous run
-of strips."""
+of strips; and that the batch's XCD-aware item order (batch_item_order) lists every item of
+200 seeded random instance lists once, in priority order within each XCD, with balanced
+loads."""
 import os
 import shutil
 import subprocess

@@ -1194,6 +1194,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
 template <class PR, int RECON, int SCHEME, int FLUX, bool PF>
 WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     const int i = B.item_inst[item];
+    if (i < 0) return;  // a padding entry of the XCD-aware order
     const BatchInst &I = B.inst[i];
     StepCtl *ctl = I.ctl;
     StageArgs A;
@@ -1229,7 +1230,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.faces_ring = 0;
     A.theta_out = nullptr;
     A.theta_flag = nullptr;
-    stage_body<PR, RECON, SCHEME, FLUX, PF>(A, item - I.item_begin, L);
+    stage_body<PR, RECON, SCHEME, FLUX, PF>(A, B.item_strip[item], L);
 }
 
 template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST>

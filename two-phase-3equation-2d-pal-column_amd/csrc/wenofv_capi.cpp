@@ -570,15 +570,19 @@ int batch_layout(wfv_batch *b) {
     std::vector<int> order(b->n);
     for (int i = 0; i < b->n; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return b->ty[p] > b->ty[q]; });
-    std::vector<int> item_inst;
-    for (int i : order) {
+    std::vector<int> nstrip(b->n), rows(b->n);
+    for (int i = 0; i < b->n; ++i) {
         BatchInst &I = b->hinst[i];
         I.ty = b->ty[i];
         I.sp = strip_plan(I.nx, I.ny, I.ty, strip_halo(b->opt.scheme, b->opt.reconstruction));
-        I.item_begin = (int)item_inst.size();
-        for (int k = 0; k < I.sp.total; ++k) item_inst.push_back(i);
+        nstrip[i] = I.sp.total;
+        rows[i] = I.ty;
     }
-    b->total_items = (int)item_inst.size();
+    // runs of 16 strips: a few row bands of one instance per XCD, while the XCDs' loads and
+    // each one's longest-first start stay as balanced as the one global order's
+    std::vector<int> item_inst, item_strip;
+    b->total_items = batch_item_order(order, nstrip, rows, 16, item_inst, item_strip);
+    item_inst.insert(item_inst.end(), item_strip.begin(), item_strip.end());
     (void)hipFree(b->d_item_inst);
     b->d_item_inst = nullptr;
     BHIPCHK(b, hipMalloc(&b->d_item_inst, item_inst.size() * sizeof(int)));
@@ -609,6 +613,7 @@ int batch_enqueue_step(wfv_batch *b) {
         BatchStageArgs B{};
         B.inst = b->d_inst;
         B.item_inst = b->d_item_inst;
+        B.item_strip = b->d_item_inst + b->total_items;
         B.total_items = b->total_items;
         B.in = pl[s].in;
         B.base = pl[s].base;

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -52,7 +53,7 @@ struct wfv_batch {
     double *ghost = nullptr;            // fixed ghost rings of the boundary-3 instances
     wfv::StepCtl *d_ctl = nullptr;
     wfv::BatchInst *d_inst = nullptr;
-    int *d_item_inst = nullptr;
+    int *d_item_inst = nullptr;         // [total_items] instance (-1: empty), then [total_items] strip
     hipStream_t stream = nullptr, own_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int persistent = 0;                 // schedule: 0 one wavefront per item, 1 persistent grid
@@ -61,6 +62,78 @@ struct wfv_batch {
 };
 
 namespace wfv {
+
+#ifndef WFV_BATCH_XCD
+#define WFV_BATCH_XCD 1
+#endif
+// Dispatch order of a batch's (instance, strip) items, XCD-aware.  The hardware hands
+// workgroup p to XCD p mod 8, each XCD with its own L2.  prio lists the instances in
+// priority order (rows per strip descending: the longest items start first), nstrip and rows
+// give each instance's strip count and rows per strip.  Each instance's strips, numbered along
+// row bands, are cut into runs of `chunk`; the runs go to the XCDs longest-processing-time
+// first by their work (strips x (rows + 1)); each XCD orders its items by instance priority
+// and strip, and its k-th item takes position 8k + x.  Vertically adjacent strips thus share
+// an L2 except at the ends of runs, and each XCD still starts its longest items first.
+// Positions past an XCD's last item hold instance -1 (a wavefront that returns at once).
+// With WFV_BATCH_XCD 0: the items in priority order, no padding.  Returns the item count.
+inline int batch_item_order(const std::vector<int> &prio, const std::vector<int> &nstrip, const std::vector<int> &rows,
+                            int chunk, std::vector<int> &item_inst, std::vector<int> &item_strip) {
+    item_inst.clear();
+    item_strip.clear();
+    if (!WFV_BATCH_XCD) {
+        for (int i : prio)
+            for (int s = 0; s < nstrip[i]; ++s) {
+                item_inst.push_back(i);
+                item_strip.push_back(s);
+            }
+        return (int)item_inst.size();
+    }
+    constexpr int NXCD = 8;
+    struct Run {
+        int inst, s0, s1, rank;
+        double work;
+    };
+    std::vector<Run> runs;
+    for (int r = 0; r < (int)prio.size(); ++r) {
+        const int i = prio[r];
+        for (int s = 0; s < nstrip[i]; s += chunk) {
+            const int e = s + chunk < nstrip[i] ? s + chunk : nstrip[i];
+            runs.push_back({i, s, e, r, (e - s) * (rows[i] + 1.0)});
+        }
+    }
+    std::vector<int> order(runs.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return runs[a].work > runs[b].work; });
+    double load[NXCD] = {};
+    std::vector<int> mine[NXCD];
+    for (int k : order) {
+        int x = 0;
+        for (int y = 1; y < NXCD; ++y)
+            if (load[y] < load[x]) x = y;
+        load[x] += runs[k].work;
+        mine[x].push_back(k);
+    }
+    size_t len = 0;
+    for (int x = 0; x < NXCD; ++x) {
+        std::sort(mine[x].begin(), mine[x].end(), [&](int a, int b) {
+            return runs[a].rank != runs[b].rank ? runs[a].rank < runs[b].rank : runs[a].s0 < runs[b].s0;
+        });
+        size_t n = 0;
+        for (int k : mine[x]) n += runs[k].s1 - runs[k].s0;
+        len = n > len ? n : len;
+    }
+    item_inst.assign(NXCD * len, -1);
+    item_strip.assign(NXCD * len, -1);
+    for (int x = 0; x < NXCD; ++x) {
+        size_t k = 0;
+        for (int q : mine[x])
+            for (int s = runs[q].s0; s < runs[q].s1; ++s, ++k) {
+                item_inst[NXCD * k + x] = runs[q].inst;
+                item_strip[NXCD * k + x] = s;
+            }
+    }
+    return (int)item_inst.size();
+}
 
 // RK schedule of SSP-RK1/2/3 (src/WENOFV.cpp:376-473): per stage (a, b, c) and registers
 struct StagePlan {

@@ -250,14 +250,14 @@ struct BatchInst {
     const double *ghost;         // fixed ghost ring (boundary 3) or nullptr
     int nx, ny, bc, ty;
     StripPlan sp;                // wavefronts of the grid (strip_plan)
-    int item_begin;              // first item of this instance in the flattened list
     int pad;
 };
 
 // one stage over every instance of a batch
 struct BatchStageArgs {
     const BatchInst *inst;
-    const int *item_inst;  // [total_items] instance of each item
+    const int *item_inst;  // [total_items] instance of each item (-1: none, the wavefront returns)
+    const int *item_strip; // [total_items] its strip within the instance
     int total_items;
     int in, base, out;     // RK register indices (base < 0: Un = U)
     double a, b, c;

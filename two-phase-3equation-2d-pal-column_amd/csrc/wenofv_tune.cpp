@@ -88,6 +88,7 @@ hipError_t time_batch_stage(wfv_batch *b, int trials, float *ms) {
     BatchStageArgs B{};
     B.inst = b->d_inst;
     B.item_inst = b->d_item_inst;
+    B.item_strip = b->d_item_inst + b->total_items;
     B.total_items = b->total_items;
     B.in = 0;
     B.base = -1;
