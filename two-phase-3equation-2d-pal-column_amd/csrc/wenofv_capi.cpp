@@ -581,7 +581,7 @@ int batch_layout(wfv_batch *b) {
     // runs of 16 strips: a few row bands of one instance per XCD, while the XCDs' loads and
     // each one's longest-first start stay as balanced as the one global order's
     std::vector<int> item_inst, item_strip;
-    b->total_items = batch_item_order(order, nstrip, rows, 16, item_inst, item_strip);
+    b->total_items = batch_item_order(order, nstrip, rows, WFV_BATCH_RUN, item_inst, item_strip);
     item_inst.insert(item_inst.end(), item_strip.begin(), item_strip.end());
     (void)hipFree(b->d_item_inst);
     b->d_item_inst = nullptr;

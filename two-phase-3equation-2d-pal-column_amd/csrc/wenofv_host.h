@@ -66,6 +66,9 @@ namespace wfv {
 #ifndef WFV_BATCH_XCD
 #define WFV_BATCH_XCD 1
 #endif
+#ifndef WFV_BATCH_RUN
+#define WFV_BATCH_RUN 16  // strips per run of the batch's XCD-aware order
+#endif
 // Dispatch order of a batch's (instance, strip) items, XCD-aware.  The hardware hands
 // workgroup p to XCD p mod 8, each XCD with its own L2.  prio lists the instances in
 // priority order (rows per strip descending: the longest items start first), nstrip and rows
