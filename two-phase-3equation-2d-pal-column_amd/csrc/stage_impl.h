@@ -360,8 +360,9 @@ WFV_DEV void recon7_x(const double s[7], int v, double dx4, bool all, double out
 // One wavefront's strip of one stage (the whole algorithm; see the file comment).  PF:
 // load each row of the register window one row ahead, so that its global-memory latency
 // overlaps a row of arithmetic (for kernels that run one wavefront per SIMD, where no
-// second wavefront hides it).
-template <class PR, int RECON, int SCHEME, int FLUX, bool PF = false>
+// second wavefront hides it).  GH: the grid may have fixed ghost cells (boundary 3); without
+// it the loader carries no ring addressing (fewer instructions and scalar registers).
+template <class PR, int RECON, int SCHEME, int FLUX, bool PF = false, bool GH = true>
 WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     // Reconstruction arithmetic.  The characteristic form projects with eigenvector
     // matrices whose condition number reaches ~1e8 in near-pure phases (z1 -> 0 or 1), which
@@ -438,7 +439,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     auto load_row = [&](int row, double out[4]) {
         bool mv;
         const int cy = map_index(row, ny, A.bc, mv);
-        const bool g = A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny);
+        const bool g = GH && A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny);
         const WFV_GLOBAL double *src = (const WFV_GLOBAL double *)(g ? A.ghost : A.U);
         const size_t stride = g ? ring_size(nx, ny) : nc;
         const size_t off = g ? ghost_offset(nx, ny, xc, row) : (size_t)cy * nx + cx;
@@ -1177,10 +1178,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 }
 
 // single domain: one wavefront per strip
-template <class PR, int RECON, int SCHEME, int FLUX, int MINW>
+template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool GH>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
-    stage_body<PR, RECON, SCHEME, FLUX, prefetch_rows<PR, RECON, SCHEME, MINW>()>(
+    stage_body<PR, RECON, SCHEME, FLUX, prefetch_rows<PR, RECON, SCHEME, MINW>(), GH>(
         A, xcd_strip((int)blockIdx.x, (int)gridDim.x), L);
 }
 
@@ -1191,7 +1192,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // items of the flattened (instance, strip) list; each instance keeps its own grid,
 // parameters, RK registers and device-side time step.  PERSIST: a grid smaller than the
 // item list strides over it (a persistent launch); otherwise one wavefront per item.
-template <class PR, int RECON, int SCHEME, int FLUX, bool PF>
+template <class PR, int RECON, int SCHEME, int FLUX, bool PF, bool GH>
 WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     const int i = B.item_inst[item];
     if (i < 0) return;  // a padding entry of the XCD-aware order
@@ -1230,18 +1231,18 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.faces_ring = 0;
     A.theta_out = nullptr;
     A.theta_flag = nullptr;
-    stage_body<PR, RECON, SCHEME, FLUX, PF>(A, B.item_strip[item], L);
+    stage_body<PR, RECON, SCHEME, FLUX, PF, GH>(A, B.item_strip[item], L);
 }
 
-template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST>
+template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST, bool GH>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_stage_kernel(
     BatchStageArgs B) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
     if constexpr (PERSIST) {
         for (int item = (int)blockIdx.x; item < B.total_items; item += (int)gridDim.x)
-            batch_item<PR, RECON, SCHEME, FLUX, MINW == 1>(B, item, L);
+            batch_item<PR, RECON, SCHEME, FLUX, MINW == 1, GH>(B, item, L);
     } else {
-        if ((int)blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX, MINW == 1>(B, (int)blockIdx.x, L);
+        if ((int)blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX, MINW == 1, GH>(B, (int)blockIdx.x, L);
     }
 }
 

@@ -622,6 +622,7 @@ int batch_enqueue_step(wfv_batch *b) {
         B.b = pl[s].b;
         B.c = pl[s].c;
         B.bp = b->opt.pp_limiter;
+        B.ghost = b->ghost != nullptr;
         B.want_dtmin = (s == ns - 1 && b->opt.rk_method != 0) ? 1 : 0;
         BHIPCHK(b, launch_batch_stage(B, b->opt.precision, b->opt.reconstruction, b->opt.scheme,
                                       b->opt.riemann_flux, b->grid, b->stream));

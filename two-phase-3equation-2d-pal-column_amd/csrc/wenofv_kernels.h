@@ -262,6 +262,7 @@ struct BatchStageArgs {
     int in, base, out;     // RK register indices (base < 0: Un = U)
     double a, b, c;
     int bp, want_dtmin;
+    int ghost;             // some instance has fixed ghost cells (boundary 3)
 };
 
 hipError_t launch_stage(const StageArgs &A, int precision, int recon, int scheme, hipStream_t s);

@@ -97,6 +97,7 @@ hipError_t time_batch_stage(wfv_batch *b, int trials, float *ms) {
     B.b = 0.0;
     B.c = 1.0;
     B.bp = b->opt.pp_limiter;
+    B.ghost = b->ghost != nullptr;
     B.want_dtmin = 0;
     return time_launches(b->stream, trials, [&] {
         return launch_batch_stage(B, b->opt.precision, b->opt.reconstruction, b->opt.scheme, b->opt.riemann_flux,
