@@ -1227,6 +1227,7 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.sp = I.sp;
     A.flags = &ctl->flags;
     A.dtmin = B.want_dtmin ? &ctl->dtmin_bits : nullptr;
+    A.ctl = nullptr;  // the batch runs its step control as a launch of its own
     A.faces_out = nullptr;
     A.faces_ring = 0;
     A.theta_out = nullptr;
