@@ -361,8 +361,10 @@ WFV_DEV void recon7_x(const double s[7], int v, double dx4, bool all, double out
 // load each row of the register window one row ahead, so that its global-memory latency
 // overlaps a row of arithmetic (for kernels that run one wavefront per SIMD, where no
 // second wavefront hides it).  GH: the grid may have fixed ghost cells (boundary 3); without
-// it the loader carries no ring addressing (fewer instructions and scalar registers).
-template <class PR, int RECON, int SCHEME, int FLUX, bool PF = false, bool GH = true>
+// it the loader carries no ring addressing (fewer instructions and scalar registers).  BPK: the
+// limiter is known to be on, so every row reconstructs every node row and the row masks,
+// their ballots and the node values of skipped rows compile away.
+template <class PR, int RECON, int SCHEME, int FLUX, bool PF = false, bool GH = true, bool BPK = false>
 WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     // Reconstruction arithmetic.  The characteristic form projects with eigenvector
     // matrices whose condition number reaches ~1e8 in near-pure phases (z1 -> 0 or 1), which
@@ -413,7 +415,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     const bool recon_lane = j >= R && j <= segw - 1 - R;
     const bool xface_lane = j >= R && j <= segw - 2 - R;
     const Phys P = A.P;
-    const bool bp = A.bp != 0;
+    const bool bp = BPK || A.bp != 0;
 
     // RK coefficients; c*dt formed as the reference does (c * deltaT, src/WENOFV.cpp:471)
     const double dt = A.dt_ptr ? *A.dt_ptr : A.dt;
@@ -504,7 +506,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         // this row's height, loaded here so that its latency overlaps the sweeps
         const double dy_row = (EARLY && it >= 1 && it <= nrow) ? ((const WFV_GLOBAL double *)A.dyC)[r] : 1.0;
         const unsigned rows =
-            !TRIM ? 0xFu
+            (!TRIM || BPK) ? 0xFu
                   : wave_or4(!live ? 0u : (full_l ? 0xFu : ((it == 0 ? 0x8u : 0u) | (it == nrow + 1 ? 0x1u : 0u))));
         if (it > 0) {
             if constexpr (SLIM) {
@@ -1178,10 +1180,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 }
 
 // single domain: one wavefront per strip
-template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool GH>
+template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool GH, bool BPK>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void stage_kernel(StageArgs A) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
-    stage_body<PR, RECON, SCHEME, FLUX, prefetch_rows<PR, RECON, SCHEME, MINW>(), GH>(
+    stage_body<PR, RECON, SCHEME, FLUX, prefetch_rows<PR, RECON, SCHEME, MINW>(), GH, BPK>(
         A, xcd_strip((int)blockIdx.x, (int)gridDim.x), L);
 }
 
@@ -1192,7 +1194,7 @@ __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) v
 // items of the flattened (instance, strip) list; each instance keeps its own grid,
 // parameters, RK registers and device-side time step.  PERSIST: a grid smaller than the
 // item list strides over it (a persistent launch); otherwise one wavefront per item.
-template <class PR, int RECON, int SCHEME, int FLUX, bool PF, bool GH>
+template <class PR, int RECON, int SCHEME, int FLUX, bool PF, bool GH, bool BPK>
 WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RECON, SCHEME>> &L) {
     const int i = B.item_inst[item];
     if (i < 0) return;  // a padding entry of the XCD-aware order
@@ -1232,18 +1234,19 @@ WFV_DEV void batch_item(const BatchStageArgs &B, int item, WaveLds<ReconT<PR, RE
     A.faces_ring = 0;
     A.theta_out = nullptr;
     A.theta_flag = nullptr;
-    stage_body<PR, RECON, SCHEME, FLUX, PF, GH>(A, B.item_strip[item], L);
+    stage_body<PR, RECON, SCHEME, FLUX, PF, GH, BPK>(A, B.item_strip[item], L);
 }
 
-template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST, bool GH>
+template <class PR, int RECON, int SCHEME, int FLUX, int MINW, bool PERSIST, bool GH, bool BPK>
 __global__ __launch_bounds__(LANES) __attribute__((amdgpu_waves_per_eu(MINW))) void batch_stage_kernel(
     BatchStageArgs B) {
     __shared__ WaveLds<ReconT<PR, RECON, SCHEME>> L;
     if constexpr (PERSIST) {
         for (int item = (int)blockIdx.x; item < B.total_items; item += (int)gridDim.x)
-            batch_item<PR, RECON, SCHEME, FLUX, MINW == 1, GH>(B, item, L);
+            batch_item<PR, RECON, SCHEME, FLUX, MINW == 1, GH, BPK>(B, item, L);
     } else {
-        if ((int)blockIdx.x < B.total_items) batch_item<PR, RECON, SCHEME, FLUX, MINW == 1, GH>(B, (int)blockIdx.x, L);
+        if ((int)blockIdx.x < B.total_items)
+            batch_item<PR, RECON, SCHEME, FLUX, MINW == 1, GH, BPK>(B, (int)blockIdx.x, L);
     }
 }
 

@@ -3,22 +3,30 @@
 #pragma once
 #include "stage_impl.h"
 
-#ifndef WFV_GH_SPLIT
-#define WFV_GH_SPLIT 1
+// Kernel variants per form.  A unit that defines WFV_SPECIALISE 0 (the rounding-control
+// precisions) has the general kernel only; the others add two: without the loader of the
+// fixed ghost ring (boundary 3), with the limiter's flag at run time or known to be on.
+#ifndef WFV_SPECIALISE
+#define WFV_SPECIALISE 1
 #endif
 
 namespace wfv {
 
-// one wavefront per workgroup: waves spread evenly over the CUs; the loader addresses a ring
-// of fixed ghost cells only where the grid has one (boundary 3)
+// one wavefront per workgroup: waves spread evenly over the CUs
+template <class PR, int MINW, int RECON, int SCHEME, int FLUX, bool GH, bool BPK>
+hipError_t launch_stage_v(const StageArgs &A, hipStream_t s) {
+    hipLaunchKernelGGL((stage_kernel<PR, RECON, SCHEME, FLUX, MINW, GH, BPK>), dim3(A.sp.total), dim3(LANES), 0, s, A);
+    return hipGetLastError();
+}
+
 template <class PR, int MINW, int RECON, int SCHEME, int FLUX>
 hipError_t launch_stage_t(const StageArgs &A, hipStream_t s) {
-    const int nstrip = A.sp.total;
-    if (A.bc == 3 || !WFV_GH_SPLIT)
-        hipLaunchKernelGGL((stage_kernel<PR, RECON, SCHEME, FLUX, MINW, true>), dim3(nstrip), dim3(LANES), 0, s, A);
-    else
-        hipLaunchKernelGGL((stage_kernel<PR, RECON, SCHEME, FLUX, MINW, false>), dim3(nstrip), dim3(LANES), 0, s, A);
-    return hipGetLastError();
+    if (!WFV_SPECIALISE || A.bc == 3) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, true, false>(A, s);
+    if constexpr (WFV_SPECIALISE) {
+        if (A.bp) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(A, s);
+        return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, false>(A, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <class PR, int MINW, int RECON, int SCHEME>
@@ -27,22 +35,27 @@ hipError_t launch_stage_f(const StageArgs &A, hipStream_t s) {
                        : launch_stage_t<PR, MINW, RECON, SCHEME, 0>(A, s);
 }
 
-template <class PR, int MINW, int RECON, int SCHEME, int FLUX, bool GH>
-hipError_t launch_batch_g(const BatchStageArgs &B, int grid, hipStream_t s) {
+template <class PR, int MINW, int RECON, int SCHEME, int FLUX, bool GH, bool BPK>
+hipError_t launch_batch_v(const BatchStageArgs &B, int grid, hipStream_t s) {
     if (grid < B.total_items)  // persistent: the wavefronts stride over the items
-        hipLaunchKernelGGL((batch_stage_kernel<PR, RECON, SCHEME, FLUX, MINW, true, GH>), dim3(grid), dim3(LANES), 0, s,
-                           B);
+        hipLaunchKernelGGL((batch_stage_kernel<PR, RECON, SCHEME, FLUX, MINW, true, GH, BPK>), dim3(grid), dim3(LANES),
+                           0, s, B);
     else
-        hipLaunchKernelGGL((batch_stage_kernel<PR, RECON, SCHEME, FLUX, MINW, false, GH>), dim3(B.total_items),
+        hipLaunchKernelGGL((batch_stage_kernel<PR, RECON, SCHEME, FLUX, MINW, false, GH, BPK>), dim3(B.total_items),
                            dim3(LANES), 0, s, B);
     return hipGetLastError();
 }
 
-// the ghost-ring loader only for a batch with a boundary-3 instance
+// a batch takes the ghost-ring loader if any instance has boundary 3; its limiter flag is
+// the batch's own
 template <class PR, int MINW, int RECON, int SCHEME, int FLUX>
 hipError_t launch_batch_t(const BatchStageArgs &B, int grid, hipStream_t s) {
-    return (B.ghost || !WFV_GH_SPLIT) ? launch_batch_g<PR, MINW, RECON, SCHEME, FLUX, true>(B, grid, s)
-                   : launch_batch_g<PR, MINW, RECON, SCHEME, FLUX, false>(B, grid, s);
+    if (!WFV_SPECIALISE || B.ghost) return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, true, false>(B, grid, s);
+    if constexpr (WFV_SPECIALISE) {
+        if (B.bp) return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(B, grid, s);
+        return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, false, false>(B, grid, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <class PR, int MINW, int RECON, int SCHEME>

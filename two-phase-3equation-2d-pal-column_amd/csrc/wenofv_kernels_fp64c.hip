@@ -4,6 +4,8 @@
 // fuse into one rounding, so the results are FP64-accurate without being bit-identical to
 // the reference: a solver that differs from the bit-faithful one by rounding alone, at every
 // step.  PrecFP64C is a type of its own so that its kernels are distinct instantiations.
+// a rounding control: the general kernel variant only
+#define WFV_SPECIALISE 0
 #include "stage_launch.h"
 
 #ifndef WFV_FP64_MINW
