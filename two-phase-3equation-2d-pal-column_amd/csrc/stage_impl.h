@@ -676,16 +676,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
 
         // ---- 2. x-direction reconstruction, back-conversion and limiting of cell (xc, r)
         //      (src/WENOFV.cpp:690-722, 836-880; limiter 1895-2000)
-        FS fw, fe, fs, fn;  // west/east/south/north face values
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                fw.set(v, g, 0.0, 0.0);
-                fe.set(v, g, 0.0, 0.0);
-                fs.set(v, g, 0.0, 0.0);
-                fn.set(v, g, 0.0, 0.0);
-            }
+        // West/east/south/north face values: set by the lanes that reconstruct (recon_lane) and
+        // read only by them (fluxes, limiter, outputs) and, through L.west and L.top, only from
+        // them; the stencil-halo lanes leave them unset and never read them.
+        FS fw, fe, fs, fn;
         double theta = 1.0;
         if (recon_lane) {
             // x-stencil value of node row gy, variable v, offset t (DEV: deviation from Bc)
@@ -1055,10 +1049,12 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             if (interior_row) {
                 // ---- 4. face east of cell (xc, r): right face | neighbour lane's left face
                 wave_sync();  // every lane has read the y-sweep values the west faces overwrite
+                if (recon_lane) {  // the lanes whose west faces a face lane (lane - 1) reads
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
+                    for (int v = 0; v < 4; ++v)
 #pragma unroll
-                    for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
+                        for (int g = 0; g < 4; ++g) L.west[v * 4 + g][lane] = fw.get(v, g, Uc[v]);
+                }
                 wave_sync();
                 dyr = EARLY ? dy_row : ((const WFV_GLOBAL double *)A.dyC)[r];
                 if constexpr (!EX) idyr = rcp_nr(dyr);
@@ -1148,10 +1144,12 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             idy_prev = idyr;
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            top_base[v] = Uc[v];
+        for (int v = 0; v < 4; ++v) top_base[v] = Uc[v];
+        if (recon_lane) {  // the next row's south faces of the owned cells (recon lanes) read them
 #pragma unroll
-            for (int g = 0; g < 4; ++g) L.top[v * 4 + g][lane] = fn.get(v, g, Uc[v]);
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) L.top[v * 4 + g][lane] = fn.get(v, g, Uc[v]);
         }
     }
 
