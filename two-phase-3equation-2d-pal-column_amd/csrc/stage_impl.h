@@ -48,6 +48,13 @@ constexpr bool prefetch_rows() {
     return MINW == 1 || (!PR::exact && SCHEME != 2 && RECON <= 1);
 }
 
+// The forms whose halo rows reconstruct only the node row they lend when the limiter is off
+// (the 5-point conservative and characteristic forms); the others reconstruct every row.
+template <int RECON, int SCHEME>
+constexpr bool trims_rows() {
+    return Stencil<SCHEME, RECON>::R == 2 && RECON <= 1;
+}
+
 WFV_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -386,7 +393,7 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     // put a 6e-6 relative error into the pressure of every flux there.  The primitive forms
     // reconstruct the pressure itself and convert back in double, so they keep W throughout.
     constexpr bool HYB = DEV && RECON == 0 && !std::is_same<W, double>::value;
-    constexpr bool TRIM = R == 2 && RECON <= 1;
+    constexpr bool TRIM = trims_rows<RECON, SCHEME>();
     // EARLY (mixed policy): a row's height and its RK base state are loaded well before their
     // use, so that their latency overlaps arithmetic; the all-FP64 kernels, whose registers
     // are spent, load them where they are used
@@ -679,7 +686,20 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         // West/east/south/north face values: set by the lanes that reconstruct (recon_lane) and
         // read only by them (fluxes, limiter, outputs) and, through L.west and L.top, only from
         // them; the stencil-halo lanes leave them unset and never read them.
+        // (The forms that reconstruct every row, whose registers are spent, zero them: that
+        // measured fewer spills there.)
         FS fw, fe, fs, fn;
+        if constexpr (!TRIM) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    fw.set(v, g, 0.0, 0.0);
+                    fe.set(v, g, 0.0, 0.0);
+                    fs.set(v, g, 0.0, 0.0);
+                    fn.set(v, g, 0.0, 0.0);
+                }
+        }
         double theta = 1.0;
         if (recon_lane) {
             // x-stencil value of node row gy, variable v, offset t (DEV: deviation from Bc)

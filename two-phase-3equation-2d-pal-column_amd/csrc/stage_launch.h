@@ -4,8 +4,10 @@
 #include "stage_impl.h"
 
 // Kernel variants per form.  A unit that defines WFV_SPECIALISE 0 (the rounding-control
-// precisions) has the general kernel only; the others add two: without the loader of the
-// fixed ghost ring (boundary 3), with the limiter's flag at run time or known to be on.
+// precisions) has the general kernel only; the others add the kernel without the loader of
+// the fixed ghost ring (boundary 3), and the forms that trim their halo rows (trims_rows) one
+// more with the limiter known to be on.  (For the forms that reconstruct every row that
+// variant only raised spills: primitive3 at two wavefronts per SIMD 18 -> 63 registers.)
 #ifndef WFV_SPECIALISE
 #define WFV_SPECIALISE 1
 #endif
@@ -23,7 +25,8 @@ template <class PR, int MINW, int RECON, int SCHEME, int FLUX>
 hipError_t launch_stage_t(const StageArgs &A, hipStream_t s) {
     if (!WFV_SPECIALISE || A.bc == 3) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, true, false>(A, s);
     if constexpr (WFV_SPECIALISE) {
-        if (A.bp) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(A, s);
+        if constexpr (trims_rows<RECON, SCHEME>())
+            if (A.bp) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(A, s);
         return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, false>(A, s);
     }
     return hipErrorInvalidValue;
@@ -52,7 +55,8 @@ template <class PR, int MINW, int RECON, int SCHEME, int FLUX>
 hipError_t launch_batch_t(const BatchStageArgs &B, int grid, hipStream_t s) {
     if (!WFV_SPECIALISE || B.ghost) return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, true, false>(B, grid, s);
     if constexpr (WFV_SPECIALISE) {
-        if (B.bp) return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(B, grid, s);
+        if constexpr (trims_rows<RECON, SCHEME>())
+            if (B.bp) return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(B, grid, s);
         return launch_batch_v<PR, MINW, RECON, SCHEME, FLUX, false, false>(B, grid, s);
     }
     return hipErrorInvalidValue;
