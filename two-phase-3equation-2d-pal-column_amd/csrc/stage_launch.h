@@ -3,8 +3,8 @@
 #pragma once
 #include "stage_impl.h"
 
-// Kernel variants per form.  A unit that defines WFV_SPECIALISE 0 (the rounding-control
-// precisions) has the general kernel only; the others add the kernel without the loader of
+// Kernel variants per form.  A unit that defines WFV_SPECIALISE 0 (the rounding control of
+// precision 3) has the general kernel only; the others add the kernel without the loader of
 // the fixed ghost ring (boundary 3), and the forms that trim their halo rows (trims_rows) one
 // more with the limiter known to be on.  (For the forms that reconstruct every row that
 // variant only raised spills: primitive3 at two wavefronts per SIMD 18 -> 63 registers.)

@@ -2,8 +2,6 @@
 // reconstruction in double (PrecMixedW64) for the conservative and primitive forms.  The
 // characteristic and 7-point forms reconstruct in double under the mixed policy already, so
 // precision 2 runs their mixed kernels.  Compiled with fused multiply-add contraction.
-// a rounding control: the general kernel variant only
-#define WFV_SPECIALISE 0
 #include "stage_launch.h"
 
 // minimum wavefronts per SIMD the register allocation must allow
