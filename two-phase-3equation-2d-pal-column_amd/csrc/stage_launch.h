@@ -14,6 +14,14 @@
 
 namespace wfv {
 
+// whether a single-domain form has the limiter-on variant: the forms that trim their halo
+// rows, except under precision 2, whose conservative kernel then spills 45 registers against
+// none and runs slower (the batch kernel of precision 2 spills 24 and still gains: it keeps it)
+template <class PR, int RECON, int SCHEME>
+constexpr bool limiter_variant() {
+    return trims_rows<RECON, SCHEME>() && !std::is_same<PR, PrecMixedW64>::value;
+}
+
 // one wavefront per workgroup: waves spread evenly over the CUs
 template <class PR, int MINW, int RECON, int SCHEME, int FLUX, bool GH, bool BPK>
 hipError_t launch_stage_v(const StageArgs &A, hipStream_t s) {
@@ -25,7 +33,7 @@ template <class PR, int MINW, int RECON, int SCHEME, int FLUX>
 hipError_t launch_stage_t(const StageArgs &A, hipStream_t s) {
     if (!WFV_SPECIALISE || A.bc == 3) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, true, false>(A, s);
     if constexpr (WFV_SPECIALISE) {
-        if constexpr (trims_rows<RECON, SCHEME>())
+        if constexpr (limiter_variant<PR, RECON, SCHEME>())
             if (A.bp) return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, true>(A, s);
         return launch_stage_v<PR, MINW, RECON, SCHEME, FLUX, false, false>(A, s);
     }
