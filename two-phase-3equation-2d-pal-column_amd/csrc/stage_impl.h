@@ -445,32 +445,49 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     constexpr bool SLIM = RECON >= 2;
     double win[SLIM ? 2 : NW][4];
     double pwin[NW][4];
-    auto load_row = [&](int row, double out[4]) {
+    // load_raw: a row's four loads and whether its momenta are to be negated (slip walls
+    // negate the momentum normal to each mirror, no-slip walls both components); the signs are
+    // applied where the values are first used, so that PF's loads stay in flight through a row
+    // of arithmetic instead of being waited for at once
+    auto load_raw = [&](int row, double out[4], bool &nu, bool &nv) {
         bool mv;
         const int cy = map_index(row, ny, A.bc, mv);
         const bool g = GH && A.bc == 3 && (xc < 0 || xc >= nx || row < 0 || row >= ny);
         const WFV_GLOBAL double *src = (const WFV_GLOBAL double *)(g ? A.ghost : A.U);
         const size_t stride = g ? ring_size(nx, ny) : nc;
         const size_t off = g ? ghost_offset(nx, ny, xc, row) : (size_t)cy * nx + cx;
-        const double a = src[off], b = src[stride + off], c = src[2 * stride + off], d = src[3 * stride + off];
-        // slip walls negate the momentum normal to each mirror, no-slip walls both components
-        const bool nu = !g && (A.bc == 2 ? mu != mv : mu);
-        const bool nv = !g && (A.bc == 2 ? mu != mv : mv);
-        out[0] = a; out[1] = b; out[2] = nu ? -c : c; out[3] = nv ? -d : d;
+        out[0] = src[off]; out[1] = src[stride + off]; out[2] = src[2 * stride + off]; out[3] = src[3 * stride + off];
+        nu = !g && (A.bc == 2 ? mu != mv : mu);
+        nv = !g && (A.bc == 2 ? mu != mv : mv);
     };
+    auto set_signs = [](double out[4], bool nu, bool nv) {
+        out[2] = nu ? -out[2] : out[2];
+        out[3] = nv ? -out[3] : out[3];
+    };
+    auto load_row = [&](int row, double out[4]) {
+        bool nu, nv;
+        load_raw(row, out, nu, nv);
+        set_signs(out, nu, nv);
+    };
+    // The window of row r = y0 - 1 (the first iteration's).  PF: one row lower, in positions
+    // 1..NW-1, and every iteration, the first too, shifts the next row in from `ahead`: a loop
+    // body without a branch around the shift, so the loads into `ahead` are not waited for at
+    // a join.
+    constexpr int K0 = PF ? 1 : 0;
+    const int wrow = PF ? y0 - 2 - R : y0 - 1 - R;  // row of window position 0
 #pragma unroll
-    for (int k = 0; k < NW; ++k) {
+    for (int k = K0; k < NW; ++k) {
         if constexpr (SLIM) {
             double t[4];
-            load_row(y0 - 1 - R + k, t);
+            load_row(wrow + k, t);
             to_prim<double, RECON, !EX>(P, t, pwin[k]);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                if (k == R - 1) win[0][v] = t[v];
-                if (k == R) win[1][v] = t[v];
+                if (!PF && k == R - 1) win[0][v] = t[v];  // rows r-1 and r (PF: r-2 and r-1,
+                if (k == R) win[1][v] = t[v];             // of which the shift keeps r-1)
             }
         } else {
-            load_row(y0 - 1 - R + k, win[k]);
+            load_row(wrow + k, win[k]);
         }
     }
 
@@ -486,11 +503,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         top_base[v] = 0.0;
     }
 
-    // PF: the row the next iteration takes into the window (the same rows as without PF)
+    // PF: the row the next shift takes into the window (the same rows as without PF)
     double ahead[4];
-    if constexpr (PF) {
-        if (nrow_w >= 0) load_row(y0 + R, ahead);
-    }
+    bool ahead_nu = false, ahead_nv = false;  // its signs, applied as it enters the window
+    if constexpr (PF) load_raw(y0 - 1 + R, ahead, ahead_nu, ahead_nv);
     // mixed policy: reciprocals of the widths replace the reference's divisions by them
     const double idxc = EX ? 0.0 : 1.0 / dxc;
     const double idxw = EX ? 0.0 : 1.0 / dxw;
@@ -498,24 +514,35 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
     const double idY = EX ? 0.0 : 1.0 / A.dY;
     asm volatile("" ::: "memory");  // the loads above are issued before the flag is tested
     if (act == 0) return;
+    // PF: the first row's signs here, where the window is waited for anyway, so that at the
+    // loop head only the loads of the previous iteration are waited for
+    if constexpr (PF) {
+        set_signs(ahead, ahead_nu, ahead_nv);
+        ahead_nu = ahead_nv = false;
+    }
     // height of the previous row and (mixed) its reciprocal, for the RK completion of that row
     double dy_prev = 1.0, idy_prev = 0.0;
 
     for (int it = 0; it <= nrow_w + 1; ++it) {
         const int r = y0 - 1 + it;
         const bool live = it <= nrow + 1;  // this lane's segment still has rows
+        // this row's height (for the RK completion), loaded early so that its latency overlaps
+        // the sweeps; PF: after the shift, whose signs would otherwise wait for this load too
+        auto row_height = [&]() -> double {
+            return (EARLY && it >= 1 && it <= nrow) ? ((const WFV_GLOBAL double *)A.dyC)[r] : 1.0;
+        };
+        double dy_row = 1.0;
+        if constexpr (!PF) dy_row = row_height();
         // Node rows (bit gy) the sweeps must reconstruct: all four in an owned row; in the rows
         // below and above the segment, whose cells only lend it their north (gy = 3) or south
         // (gy = 0) face, just that row, unless the limiter needs the cells' other faces too or
         // every face value is recorded.  A union over the wavefront's lanes.
         // (The primitive and 7-point forms reconstruct every row: their register budget is spent.)
         const bool full_l = bp || A.faces_out || (it >= 1 && it <= nrow);
-        // this row's height, loaded here so that its latency overlaps the sweeps
-        const double dy_row = (EARLY && it >= 1 && it <= nrow) ? ((const WFV_GLOBAL double *)A.dyC)[r] : 1.0;
         const unsigned rows =
             (!TRIM || BPK) ? 0xFu
                   : wave_or4(!live ? 0u : (full_l ? 0xFu : ((it == 0 ? 0x8u : 0u) | (it == nrow + 1 ? 0x1u : 0u))));
-        if (it > 0) {
+        if (PF || it > 0) {
             if constexpr (SLIM) {
 #pragma unroll
                 for (int k = 0; k < NW - 1; ++k)
@@ -525,7 +552,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 if constexpr (PF) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) t[v] = ahead[v];
-                    if (it <= nrow_w) load_row(r + R + 1, ahead);
+                    set_signs(t, ahead_nu, ahead_nv);
+                    // without a branch (which would make the loaded registers wait for a
+                    // copy at its end): the last iteration loads its own last row again, unused
+                    load_raw(it <= nrow_w ? r + R + 1 : r + R, ahead, ahead_nu, ahead_nv);
                 } else {
                     load_row(r + R, t);
                 }
@@ -541,12 +571,16 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
                 if constexpr (PF) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) win[NW - 1][v] = ahead[v];
-                    if (it <= nrow_w) load_row(r + R + 1, ahead);
+                    set_signs(win[NW - 1], ahead_nu, ahead_nv);
+                    // without a branch (which would make the loaded registers wait for a
+                    // copy at its end): the last iteration loads its own last row again, unused
+                    load_raw(it <= nrow_w ? r + R + 1 : r + R, ahead, ahead_nu, ahead_nv);
                 } else {
                     load_row(r + R, win[NW - 1]);
                 }
             }
         }
+        if constexpr (PF) dy_row = row_height();
         const double *Uc;  // cell (xc, r)
         const double *Bc;  // its reconstructed variables
         if constexpr (SLIM) {
