@@ -581,6 +581,10 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
             }
         }
         if constexpr (PF) dy_row = row_height();
+        double Ub[4];  // the RK base state of row r-1, completed in this iteration
+        const bool complete = it >= 2 && owned && live;
+        if constexpr (EARLY && PF)
+            if (complete && A.Un) load_cell<double>(A.Un, nc, (size_t)(r - 1) * nx + xc, false, false, 0, Ub);
         const double *Uc;  // cell (xc, r)
         const double *Bc;  // its reconstructed variables
         if constexpr (SLIM) {
@@ -1085,9 +1089,9 @@ WFV_DEV void stage_body(const StageArgs &A, const int strip, WaveLds<ReconT<PR, 
         if (it > 0) {
             const bool interior_row = it <= nrow;
             // the RK base state of row r-1, loaded ahead of the fluxes that precede its use
-            double Ub[4];
-            const bool complete = it >= 2 && owned && live;
-            if (EARLY && complete && A.Un) load_cell<double>(A.Un, nc, (size_t)(r - 1) * nx + xc, false, false, 0, Ub);
+            // (PF: at the top of the iteration, ahead of the sweeps too)
+            if constexpr (EARLY && !PF)
+                if (complete && A.Un) load_cell<double>(A.Un, nc, (size_t)(r - 1) * nx + xc, false, false, 0, Ub);
             // ---- 3. face below row r: Riemann problem (top of r-1 | bottom of r)
             double G[4] = {0.0, 0.0, 0.0, 0.0};
             if (owned && live) {
