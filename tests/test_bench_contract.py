@@ -31,7 +31,10 @@ def test_profile_summaries_hold_the_bench_figures(name):
     for key in KEYS:
         v = bench.profiled(name, key)
         assert v is not None and v > 0, key
-    assert 0.5 < bench.profiled(name, "valu_busy_frac") <= 1.0
+    # quad-cycles with a VALU instruction executing, summed over the waves: it passes 1 when
+    # two waves' instructions overlap, up to the two resident waves per SIMD
+    assert 0.5 < bench.profiled(name, "valu_busy_frac") <= 2.0
+    assert 0.5 < bench.profiled(name, "valu_issue_frac") <= 1.0
     # the summary's algorithmic bytes are the roofline unit (85.33 B per cell-stage) over the
     # batch's cells, and its measured traffic is at least that
     cells = 14315520
